@@ -1,0 +1,273 @@
+/*
+ * s1gpu.h -- C ABI of the MI355X (gfx950) Sentinel-1 Level-0 -> SLC hot path.
+ *
+ * Plain C: pointers, sizes and POD descriptors only.  Unless stated
+ * otherwise every data pointer is a DEVICE pointer (hipMalloc'd or from
+ * another HIP allocator) and every call is asynchronous on the context's
+ * stream.  Complex arrays are interleaved (re, im), row-major, one radar
+ * line (echo / range line) per row; leading dimensions count complex
+ * elements.  Precision is chosen per call:
+ *   S1G_F64  complex128 data, FP64 arithmetic (the reference's arithmetic);
+ *   S1G_F32  complex64 data, FP32 arithmetic and accumulation, with the
+ *            replica-chirp phase, the range-Doppler phase and the RCMC
+ *            offsets computed in FP64 and reduced mod 2 pi before FP32 use.
+ *
+ * Return value: 0 on success, a negative S1G_E* code otherwise;
+ * s1g_last_error() returns a message.  No C++ exception crosses this ABI.
+ *
+ * Reference citations are file:line in AndrewPlayer3/sentinel1_decode src/.
+ */
+#ifndef S1GPU_H
+#define S1GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define S1G_ABI_VERSION 1
+
+/* precision selectors */
+#define S1G_F64 0
+#define S1G_F32 1
+
+/* return codes */
+#define S1G_OK 0
+#define S1G_EARG (-1)        /* invalid argument                        */
+#define S1G_EHIP (-2)        /* HIP runtime error                       */
+#define S1G_EUNSUPPORTED (-3) /* length/shape this build cannot handle   */
+#define S1G_ENOMEM (-4)      /* device allocation failed                */
+#define S1G_EDECODE (-5)     /* >= 1 packet failed; see per-packet status */
+
+/* per-packet decode status (status[] of s1g_decode_fdbaq) */
+#define S1G_PKT_OK 0
+#define S1G_PKT_ERR_BRC 1     /* BRC field > 4 (packet.cpp:795-798)                */
+#define S1G_PKT_ERR_OVERRUN 2 /* read past the user data (decoding_utils.cpp:107-119) */
+#define S1G_PKT_ERR_HUFFMAN 3 /* code longer than 10 bits (decoding_utils.cpp:69-72) */
+#define S1G_PKT_ERR_MCODE 4   /* m_code above the BRC flag (packet.cpp:690)         */
+#define S1G_PKT_ERR_ARG 5     /* num_quads < 0, or packet above the stated maxima    */
+
+typedef struct s1g_ctx s1g_ctx;
+
+/* ------------------------------------------------------------ lifecycle */
+/* Create a context on HIP device `device`.  A context owns a stream, the
+ * FFT plans (twiddle tables, digit-reversal maps), the cached range
+ * reference functions, a growable workspace and captured burst graphs.  A
+ * context is not thread-safe; use one per host thread. */
+int s1g_create(s1g_ctx** ctx, int device);
+int s1g_destroy(s1g_ctx* ctx);
+const char* s1g_last_error(const s1g_ctx* ctx);
+/* Run subsequent work on `stream` (a hipStream_t); NULL restores the
+ * context's own stream. */
+int s1g_set_stream(s1g_ctx* ctx, void* stream);
+int s1g_synchronize(s1g_ctx* ctx);
+
+/* ------------------------------------------------------- FDBAQ decode */
+/* Decode n_packets FDBAQ (BAQ mode 12-14, data format D) packets.
+ * Replaces L0Packet::get_signal (packet.h:205, packet.cpp:393-464) and the
+ * per-packet decode loops decode_packets / decode_packets_in_place
+ * (packet.h:228,238) and S1_Decoder::_range_compress (s1_decoder.cpp:462-468).
+ *   user_data          byte buffer holding the packets' user data fields
+ *   ud_offsets[i]      byte offset of packet i's user data in user_data
+ *   ud_lengths[i]      its length in bytes (packet_data_length + 1 - 62)
+ *   num_quads[i]       the packet's num_quadratures header field
+ *   max_user_data_bytes, max_num_quads  host-side bounds over all packets
+ *                      (they size on-chip buffers; larger packets fail
+ *                      with S1G_PKT_ERR_ARG)
+ *   out                complex [n_packets][ld_out] (F32: float2, F64:
+ *                      double2), 2*num_quads[i] samples written per row
+ *   status[i]          S1G_PKT_* per packet (rows of failed packets are
+ *                      zero-filled)
+ * Decoding is bit-exact: F64 output equals the reference's doubles, F32
+ * output is those doubles rounded once to float. */
+int s1g_decode_fdbaq(s1g_ctx* ctx, const uint8_t* user_data, const uint64_t* ud_offsets,
+                     const uint32_t* ud_lengths, const int32_t* num_quads, int n_packets,
+                     int max_user_data_bytes, int max_num_quads, int prec, void* out,
+                     int64_t ld_out, int32_t* status);
+
+/* Host function: length (bits) and magnitude code of the FDBAQ Huffman code
+ * word at the top of the 10-bit value `code10` for bit-rate code `brc`
+ * (the decoder's symbol rule; huffman_decode, decoding_utils.cpp:44-77). */
+int s1g_fdbaq_symbol(int brc, uint32_t code10, int* code_len, int* m_code);
+
+/* -------------------------------------------------- packet headers (host) */
+/* Decoded primary (6 B) and secondary (62 B) header of one space packet and
+ * the physical quantities the reference derives from them (parse_packet_header,
+ * packet.cpp:829-846; accessors packet.cpp:15-100,277-342,376-380). */
+typedef struct {
+    int64_t primary[8];     /* fields in the order of structs.h:42-51   */
+    int64_t secondary[46];  /* fields in the order of structs.h:68-115  */
+    int32_t num_quads;
+    int32_t baq_mode;
+    int32_t range_decimation;
+    int32_t rank;
+    int32_t swath_number;
+    int32_t user_data_length;     /* packet_data_length + 1 - 62                 */
+    int32_t replica_chirp_length; /* floor(fs * txpl)                            */
+    int32_t baq_block_length;     /* 8 * (field + 1)                             */
+    char data_format;             /* 'A'/'B'/'C'/'D' (packet.cpp:265-274)        */
+    char reserved[7];
+    double pri_us, pulse_length_us, swl_us, swst_us, rx_gain_db;
+    double txprr_mhz_us, txpsf_mhz, fs_mhz;
+} s1g_packet_info;
+
+/* Host function.  Returns S1G_EARG if nbytes < 68 or the BAQ mode is not a
+ * known one. */
+int s1g_parse_packet_header(const uint8_t* packet, int64_t nbytes, s1g_packet_info* info);
+/* Host functions: slant ranges (linspace over the sampling window,
+ * packet.cpp:284-301) and timing corrections (packet.cpp:322-342) of a
+ * packet's 2*num_quads range samples (num_ranges <= 0 means 2*num_quads). */
+int s1g_slant_ranges(const s1g_packet_info* info, int num_ranges, double* out);
+int s1g_timing_corrections(const s1g_packet_info* info, double* out);
+
+/* ---------------------------------------------------- range compression */
+/* Replica-chirp parameters of a burst (from its first packet:
+ * L0Packet::get_start_frequency / get_tx_ramp_rate / get_pulse_length /
+ * get_range_sample_rate, packet.cpp:36-100,277-281). */
+typedef struct {
+    int32_t n_lines;      /* echo lines (packets)                         */
+    int32_t n_samples;    /* complex samples per line, 2 * num_quads       */
+    double txpsf_mhz;     /* TXPSF, MHz                                    */
+    double txprr_mhz_us;  /* TXPRR, MHz/us                                 */
+    double txpl_us;       /* pulse length, us                              */
+    double fs_mhz;        /* range sample rate, MHz                        */
+} s1g_range_desc;
+
+/* Range-compress n_lines lines (out-of-place).  Replaces the range part of
+ * S1_Decoder::_range_compress (s1_decoder.cpp:476-497): pad to
+ * P = N + 2 nc, FFT, multiply by get_reference_function(get_replica_chirp())
+ * (image_formation.cpp:4-43, packet.cpp:345-380), IFFT, fftshift, crop.
+ * The result equals that linear correlation including its scale
+ * nc^2 * P; the FFT length used internally is chosen by the library. */
+int s1g_range_compress(s1g_ctx* ctx, const s1g_range_desc* desc, int prec, const void* in,
+                       int64_t ld_in, void* out, int64_t ld_out);
+
+/* Replica chirp length nc = floor(fs * txpl) (L0Packet::get_replica_chirp_length,
+ * packet.cpp:376-380), host function. */
+int s1g_replica_chirp_length(double fs_mhz, double txpl_us);
+
+/* ------------------------------------------------------------- azimuth */
+/* Azimuth (column) FFT of an [n_lines][n_samples] burst, in place.
+ * inverse = 0: FFT along azimuth then fftshift (s1_decoder.cpp:499-505);
+ * inverse = 1: inverse FFT times 1/n_lines, no shift
+ * (compute_axis_dft_in_place(data, 0, 0, true), s1_decoder.cpp:682,
+ * signal_processing.cpp:581-605). */
+int s1g_azimuth_fft(s1g_ctx* ctx, int prec, int n_lines, int n_samples, void* data, int64_t ld,
+                    int inverse);
+
+/* Range-Doppler processing of one burst.  Row i of the input is the
+ * fftshifted Doppler bin f_i = linspace(-prf/2, prf/2, n_lines)[i]
+ * (s1_decoder.cpp:604).  Per sample (i, j):
+ *   D  = sqrt(1 - lambda^2 (f_i + f_dc[j])^2 / (4 v_rel[j]^2))
+ *        (apply_secondary_range_compression, image_formation.cpp:439-441)
+ *   RCMC: x <- sinc_interpolate(row_i, j + (R_j / D - R_j) / range_spacing, 8)
+ *        (signal_processing.cpp:238-257 with the migration of
+ *         image_formation.cpp:469)  [if apply_rcmc]
+ *   x /= sqrt((ref_range / R_j)^3)                       (s1_decoder.cpp:643-646)
+ *   x *= exp(i 4 pi R_j D / lambda) exp(i 2 pi (f_i + f_dc[j]) t_corr[j])
+ *                                                          (s1_decoder.cpp:658-676)
+ * Arrays are device pointers of n_samples doubles. */
+typedef struct {
+    int32_t n_lines;
+    int32_t n_samples;
+    int32_t apply_rcmc;
+    int32_t reserved;
+    double prf;            /* Hz                                              */
+    double wavelength;     /* m                                               */
+    double range_spacing;  /* slant-range step of the range grid, m           */
+    double ref_range;      /* R_ref = slant_ranges[n_samples / 2], m          */
+    const double* slant_ranges; /* R_j  (L0Packet::get_slant_ranges)          */
+    const double* v_rel;        /* effective velocity per range sample, m/s   */
+    const double* f_dc;         /* Doppler centroid per range sample, Hz      */
+    const double* t_corr;       /* L0Packet::get_timing_corrections, s        */
+} s1g_azimuth_desc;
+
+int s1g_rcmc_matched_filter(s1g_ctx* ctx, const s1g_azimuth_desc* desc, int prec, const void* in,
+                            int64_t ld_in, void* out, int64_t ld_out);
+
+/* ---------------------------------------------------------------- ECCM */
+/* RFI detection and notch, in place (eccm, signal_processing.cpp:913-1004,
+ * as called from s1_decoder.cpp:470-474 with fft_size 128, stride 64).
+ * A line is processed when max |FFT_N(line)| >= detection_threshold; then
+ * each 128-sample window's spectrum bins whose |central gradient| reaches
+ * mitigation_threshold are zeroed and the windows are written back in
+ * order.  flags (device, n_lines bytes, may be NULL) receives 1 for
+ * processed lines. */
+typedef struct {
+    int32_t n_lines;
+    int32_t n_samples;
+    int32_t fft_size;
+    int32_t stride;
+    double detection_threshold;
+    double mitigation_threshold;
+} s1g_eccm_desc;
+
+int s1g_eccm(s1g_ctx* ctx, const s1g_eccm_desc* desc, int prec, void* data, int64_t ld,
+             uint8_t* flags);
+
+/* --------------------------------------------------- whole burst chain */
+/* One burst from packets to SLC: decode -> [ECCM] -> range compression ->
+ * azimuth FFT -> RCMC + matched filter -> azimuth IFFT
+ * (S1_Decoder::_azimuth_compress, s1_decoder.cpp:511-701, on its
+ * non-TOPS formulas).  Intermediate arrays live in the context workspace. */
+typedef struct {
+    const uint8_t* user_data;
+    const uint64_t* ud_offsets;
+    const uint32_t* ud_lengths;
+    const int32_t* num_quads;
+    int32_t* status;            /* n_lines per-packet decode status      */
+    int32_t max_user_data_bytes;
+    int32_t max_num_quads;
+    int32_t use_eccm;
+    int32_t reserved;
+    s1g_range_desc range;
+    s1g_azimuth_desc azimuth;
+    s1g_eccm_desc eccm;
+    uint8_t* eccm_flags;        /* may be NULL                           */
+    void* out;                  /* SLC [n_lines][ld_out]                 */
+    int64_t ld_out;
+} s1g_burst_desc;
+
+int s1g_focus_burst(s1g_ctx* ctx, const s1g_burst_desc* desc, int prec);
+
+/* Several bursts (uneven sizes allowed) in one submission.  flags:
+ *   S1G_BATCH_GRAPH  capture the whole batch as one hipGraph and replay it
+ *                    (the graph is cached per descriptor set).             */
+#define S1G_BATCH_GRAPH 1
+int s1g_focus_batch(s1g_ctx* ctx, const s1g_burst_desc* descs, int n_bursts, int prec, int flags);
+
+/* ----------------------------------------------------------- autofocus */
+/* (No reference counterpart.)  rd: range-Doppler data after RCMC and
+ * matched filter, [n_lines][ld] (rows = Doppler bins).  phase: device
+ * double[n_lines], per-row azimuth phase correction.
+ * s1g_af_forward: image = IFFT_az(rd * e^{i phase}) / n_lines written to img
+ *   (may be NULL), entropy E = -sum p log p, p = |img|^2 / sum |img|^2,
+ *   returned through *entropy (host pointer; the call synchronises).
+ * s1g_af_adjoint: additionally grad (device double[n_lines]) = dE/dphase. */
+int s1g_af_forward(s1g_ctx* ctx, int prec, int n_lines, int n_samples, const void* rd, int64_t ld,
+                   const double* phase, void* img, double* entropy);
+int s1g_af_adjoint(s1g_ctx* ctx, int prec, int n_lines, int n_samples, const void* rd, int64_t ld,
+                   const double* phase, double* entropy, double* grad);
+
+/* ------------------------------------------------------- stage timers */
+/* When enabled, every stage launch is bracketed by HIP events on the
+ * context's stream; s1g_profile_read synchronises, returns per-stage total
+ * milliseconds and launch counts (arrays of S1G_NSTAGES) and optionally
+ * resets them.  Timers are not recorded while a batch is being captured. */
+#define S1G_STAGE_DECODE 0
+#define S1G_STAGE_ECCM 1
+#define S1G_STAGE_RANGE 2
+#define S1G_STAGE_AZ_FFT 3
+#define S1G_STAGE_RCMC_MF 4
+#define S1G_STAGE_AZ_IFFT 5
+#define S1G_STAGE_FILTER 6
+#define S1G_STAGE_AUTOFOCUS 7
+#define S1G_NSTAGES 8
+int s1g_profile_enable(s1g_ctx* ctx, int on);
+int s1g_profile_read(s1g_ctx* ctx, double* ms, int64_t* launches, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* S1GPU_H */

@@ -1,0 +1,319 @@
+"""CPU FP64 oracle for the Sentinel-1 L0 -> SLC hot path.
+
+TEST INFRASTRUCTURE ONLY.  This package is the parity checker and the timed
+CPU baseline.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import it; the product package never does.
+
+Thin numpy/ctypes wrapper over ``libs1oracle.so`` (built from
+``s1_oracle.c`` by ``oracle/Makefile``).  Each wrapper names the reference
+function (AndrewPlayer3/sentinel1_decode ``src/`` file:line) it restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle library (and the reference decode shim where the
+    reference sources exist)."""
+    so = os.path.join(_HERE, "libs1oracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "libs1oracle.so")
+        if not os.path.exists(so):
+            build()
+        L = C.CDLL(so)
+        L.orc_read_n_bits.argtypes = [_u8, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_huffman_decode.argtypes = [_u8, C.c_int64, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        L.orc_s_value_d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.orc_s_value_d.restype = C.c_double
+        L.orc_decode_packet_d.argtypes = [_u8, C.c_int64, C.c_int, _dp, _u8, _u8]
+        L.orc_decode_packets.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                         C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_parse_headers.argtypes = [_u8, C.c_int64, C.c_void_p, C.c_void_p]
+        L.orc_packet_params.argtypes = [C.c_void_p, _dp]
+        L.orc_slant_ranges.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_timing_corrections.argtypes = [C.c_void_p, _dp]
+        L.orc_linspace.argtypes = [C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_fft.argtypes = [_dp, C.c_int, C.c_int]
+        L.orc_fft_many.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_int]
+        L.orc_replica_chirp_length.argtypes = [C.c_double, C.c_double]
+        L.orc_replica_chirp.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.orc_reference_function.argtypes = [_dp, C.c_int, _dp]
+        L.orc_range_compress.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double,
+                                         C.c_double, C.c_double]
+        L.orc_azimuth_fft.argtypes = [_dp, C.c_int, C.c_int, C.c_int64]
+        L.orc_azimuth_ifft.argtypes = [_dp, C.c_int, C.c_int, C.c_int64]
+        L.orc_rcmc_mf.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                  _dp, _dp, _dp, _dp, C.c_int]
+        L.orc_sinc_interpolate.argtypes = [_dp, C.c_int, C.c_double, C.c_int, _dp]
+        L.orc_eccm.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, _u8]
+        L.orc_af_entropy_grad.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, _dp, _dp]
+        L.orc_af_entropy_grad.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _table_call(fn):
+    out = dict(thidx=np.zeros(5, np.int32), mflag=np.zeros(5, np.int32), simple=np.zeros(50), nrl=np.zeros(80),
+               sf=np.zeros(256), rgdec=np.zeros(12), consts=np.zeros(8), huff=np.zeros(160, np.int32),
+               huff_n=np.zeros(5, np.int32), huff_start=np.zeros(5, np.int32))
+    fn(*[v.ctypes.data for v in out.values()])
+    out["simple"] = out["simple"].reshape(5, 10)
+    out["nrl"] = out["nrl"].reshape(5, 16)
+    h = out["huff"].reshape(5, 16, 2)
+    out["huff"] = {b: sorted((int(h[b, k, 0]), int(h[b, k, 1])) for k in range(out["huff_n"][b])) for b in range(5)}
+    return out
+
+
+def tables():
+    """The oracle's decode tables and constants."""
+    L = lib()
+    L.orc_tables.argtypes = [C.c_void_p] * 10
+    return _table_call(L.orc_tables)
+
+
+def ref_tables():
+    """The reference's own tables (structs.h) via the shim, or None."""
+    R = ref_lib()
+    if R is None:
+        return None
+    R.ref_tables.argtypes = [C.c_void_p] * 10
+    return _table_call(R.ref_tables)
+
+
+def ref_parse_headers(packet: bytes):
+    """The reference's parse_packet_header over its own width tables, or None."""
+    R = ref_lib()
+    if R is None:
+        return None
+    R.ref_parse_headers.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    d = np.frombuffer(bytes(packet[:68]), np.uint8).copy()
+    p = np.zeros(8, np.int64)
+    s = np.zeros(46, np.int64)
+    if R.ref_parse_headers(d.ctypes.data, p.ctypes.data, s.ctypes.data):
+        raise ValueError("reference header parse failed")
+    return p, s
+
+
+def ref_lib():
+    """The reference's own decoding_utils.cpp behind a C shim, or None where it
+    was not built (the reference tree is absent)."""
+    global _REF
+    if _REF is None:
+        so = os.path.join(_HERE, "_ref", "libref_decoding.so")
+        if not os.path.exists(so):
+            return None
+        R = C.CDLL(so)
+        R.ref_read_n_bits.argtypes = [_u8, C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
+        R.ref_huffman_decode.argtypes = [_u8, C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _REF = R
+    return _REF
+
+
+def _c128(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+def _as_f64(a):
+    return a.view(np.float64).reshape(-1) if a.size else np.zeros(2)
+
+
+# ---------------------------------------------------------------- decoding
+def read_n_bits(data: bytes, start_bit: int, n: int):
+    """decoding_utils.cpp:95-134; returns (status, value)."""
+    d = np.frombuffer(bytes(data) or b"\0", dtype=np.uint8).copy()
+    out = C.c_uint64(0)
+    st = lib().orc_read_n_bits(d, len(data), start_bit, n, C.byref(out))
+    return st, out.value
+
+
+def huffman_decode(data: bytes, brc: int, bit_index: int):
+    """decoding_utils.cpp:44-77; returns (status, m_code, new_bit_index)."""
+    d = np.frombuffer(bytes(data) or b"\0", dtype=np.uint8).copy()
+    bi = C.c_int64(bit_index)
+    m = C.c_int(-1)
+    st = lib().orc_huffman_decode(d, len(data), brc, C.byref(bi), C.byref(m))
+    return st, m.value, bi.value
+
+
+def s_value(brc: int, thidx: int, sign: int, m_code: int):
+    """packet.cpp:678-693; returns (status, value)."""
+    st = C.c_int(0)
+    v = lib().orc_s_value_d(brc, thidx, sign, m_code, C.byref(st))
+    return st.value, v
+
+
+def decode_packet(user_data: bytes, num_quads: int):
+    """packet.cpp:404-453,696-818 -> (status, complex128[2*NQ], brc[NB], thidx[NB])."""
+    nb = -(-2 * num_quads // 256) if num_quads > 0 else 0
+    d = np.frombuffer(bytes(user_data) or b"\0", dtype=np.uint8).copy()
+    out = np.zeros(max(4 * num_quads, 2), dtype=np.float64)
+    brc = np.zeros(max(nb, 1), np.uint8)
+    thr = np.zeros(max(nb, 1), np.uint8)
+    st = lib().orc_decode_packet_d(d, len(user_data), num_quads, out, brc, thr)
+    sig = out[: 4 * num_quads].view(np.complex128).copy()
+    return st, sig, brc[:nb].copy(), thr[:nb].copy()
+
+
+def decode_packets(stream: np.ndarray, offsets, lengths, num_quads, n_samples=None):
+    """Batch decode (s1_decoder.cpp:462-468) -> (complex128[L, N], status[L])."""
+    stream = np.ascontiguousarray(stream, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    nq = np.ascontiguousarray(num_quads, dtype=np.int32)
+    L = len(offsets)
+    N = int(n_samples if n_samples is not None else 2 * int(nq.max(initial=0)))
+    out = np.zeros((L, max(N, 1)), np.complex128)
+    status = np.zeros(L, np.int32)
+    lib().orc_decode_packets(stream.ctypes.data, offsets.ctypes.data, lengths.ctypes.data, nq.ctypes.data,
+                             L, out.ctypes.data, out.shape[1], status.ctypes.data)
+    return out[:, :N], status
+
+
+# ---------------------------------------------------------------- headers
+def parse_headers(packet: bytes):
+    """parse_packet_header (packet.cpp:829-846) over the 6 + 62 header bytes."""
+    d = np.frombuffer(bytes(packet), dtype=np.uint8).copy()
+    p = np.zeros(8, np.int64)
+    s = np.zeros(46, np.int64)
+    st = lib().orc_parse_headers(d, len(d), p.ctypes.data, s.ctypes.data)
+    if st:
+        raise ValueError("packet shorter than its headers")
+    return p, s
+
+
+def packet_params(sec: np.ndarray):
+    """packet.cpp:29-100,277-281,376-380 as a dict."""
+    sec = np.ascontiguousarray(sec, dtype=np.int64)
+    out = np.zeros(10)
+    lib().orc_packet_params(sec.ctypes.data, out)
+    keys = ["pri_us", "pulse_length_us", "swl_us", "swst_us", "rx_gain_db", "txprr", "txpsf", "fs_mhz",
+            "replica_chirp_length", "baq_block_length"]
+    return dict(zip(keys, out.tolist()))
+
+
+def slant_ranges(sec, num_ranges=0):
+    sec = np.ascontiguousarray(sec, dtype=np.int64)
+    n = num_ranges if num_ranges > 0 else 2 * int(sec[44])
+    out = np.zeros(n)
+    lib().orc_slant_ranges(sec.ctypes.data, n, out)
+    return out
+
+
+def timing_corrections(sec):
+    sec = np.ascontiguousarray(sec, dtype=np.int64)
+    out = np.zeros(2 * int(sec[44]))
+    lib().orc_timing_corrections(sec.ctypes.data, out)
+    return out
+
+
+def linspace(start, end, size):
+    out = np.zeros(size)
+    lib().orc_linspace(start, end, size, out)
+    return out
+
+
+# ---------------------------------------------------------------- DSP
+def fft(x, inverse=False):
+    """Unnormalised DFT with FFTW's sign conventions (signal_processing.cpp:516-578)."""
+    y = _c128(x).copy()
+    lib().orc_fft(_as_f64(y), y.size, int(inverse))
+    return y
+
+
+def fft_rows(x, inverse=False):
+    y = np.ascontiguousarray(np.array(x, dtype=np.complex128, copy=True))
+    lib().orc_fft_many(_as_f64(y), y.shape[1], y.shape[0], y.shape[1], int(inverse))
+    return y
+
+
+def replica_chirp_length(fs_mhz, txpl_us):
+    return lib().orc_replica_chirp_length(fs_mhz, txpl_us)
+
+
+def replica_chirp(txpsf, txprr, txpl, fs_mhz, num_range):
+    nc = replica_chirp_length(fs_mhz, txpl)
+    out = np.zeros(num_range + 2 * nc, np.complex128)
+    lib().orc_replica_chirp(txpsf, txprr, txpl, fs_mhz, num_range, _as_f64(out))
+    return out
+
+
+def reference_function(chirp):
+    chirp = _c128(chirp)
+    out = np.zeros_like(chirp)
+    lib().orc_reference_function(_as_f64(chirp), chirp.size, _as_f64(out))
+    return out
+
+
+def range_compress(data, txpsf, txprr, txpl, fs_mhz):
+    """_range_compress without ECCM (s1_decoder.cpp:476-497), returns a copy."""
+    y = np.ascontiguousarray(np.array(data, dtype=np.complex128, copy=True))
+    L, N = y.shape
+    lib().orc_range_compress(_as_f64(y), L, N, N, txpsf, txprr, txpl, fs_mhz)
+    return y
+
+
+def azimuth_fft(data):
+    y = np.ascontiguousarray(np.array(data, dtype=np.complex128, copy=True))
+    lib().orc_azimuth_fft(_as_f64(y), y.shape[0], y.shape[1], y.shape[1])
+    return y
+
+
+def azimuth_ifft(data):
+    y = np.ascontiguousarray(np.array(data, dtype=np.complex128, copy=True))
+    lib().orc_azimuth_ifft(_as_f64(y), y.shape[0], y.shape[1], y.shape[1])
+    return y
+
+
+def sinc_interpolate(row, t, half_width=8):
+    row = _c128(row)
+    out = np.zeros(2)
+    lib().orc_sinc_interpolate(_as_f64(row), row.size, float(t), half_width, out)
+    return complex(out[0], out[1])
+
+
+def rcmc_mf(data, prf, range_spacing, ref_range, slant_ranges_, v_rel, f_dc, t_corr, apply_rcmc=True):
+    y = np.ascontiguousarray(np.array(data, dtype=np.complex128, copy=True))
+    L, N = y.shape
+    f = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (N,)))
+    lib().orc_rcmc_mf(_as_f64(y), L, N, N, prf, range_spacing, ref_range, f(slant_ranges_), f(v_rel), f(f_dc),
+                      f(t_corr), int(apply_rcmc))
+    return y
+
+
+def eccm(data, det, mit, fft_size=128, stride=64):
+    """signal_processing.cpp:913-1004 -> (cleaned copy, per-line detection flags)."""
+    y = np.ascontiguousarray(np.array(data, dtype=np.complex128, copy=True))
+    L, N = y.shape
+    flags = np.zeros(L, np.uint8)
+    lib().orc_eccm(_as_f64(y), L, N, N, fft_size, stride, float(det), float(mit), flags)
+    return y, flags
+
+
+def af_entropy_grad(rd, phase):
+    rd = np.ascontiguousarray(rd, dtype=np.complex128)
+    L, N = rd.shape
+    ph = np.ascontiguousarray(phase, dtype=np.float64)
+    g = np.zeros(L)
+    e = lib().orc_af_entropy_grad(_as_f64(rd), L, N, N, ph, g)
+    return e, g

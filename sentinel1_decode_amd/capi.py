@@ -1,0 +1,280 @@
+"""ctypes binding of the C ABI in include/s1gpu.h (libs1gpu.so).
+
+Device buffers are torch tensors (torch is used for device memory and the
+stream only); every call runs on torch's current stream.  The library must be
+built (``python -m sentinel1_decode_amd.build``); a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libs1gpu.so")
+
+S1G_F64 = 0
+S1G_F32 = 1
+S1G_OK = 0
+S1G_BATCH_GRAPH = 1
+PKT_STATUS = {0: "ok", 1: "brc", 2: "overrun", 3: "huffman", 4: "mcode", 5: "arg"}
+
+
+class RangeDesc(C.Structure):
+    _fields_ = [("n_lines", C.c_int32), ("n_samples", C.c_int32), ("txpsf_mhz", C.c_double),
+                ("txprr_mhz_us", C.c_double), ("txpl_us", C.c_double), ("fs_mhz", C.c_double)]
+
+
+class AzimuthDesc(C.Structure):
+    _fields_ = [("n_lines", C.c_int32), ("n_samples", C.c_int32), ("apply_rcmc", C.c_int32),
+                ("reserved", C.c_int32), ("prf", C.c_double), ("wavelength", C.c_double),
+                ("range_spacing", C.c_double), ("ref_range", C.c_double), ("slant_ranges", C.c_void_p),
+                ("v_rel", C.c_void_p), ("f_dc", C.c_void_p), ("t_corr", C.c_void_p)]
+
+
+class EccmDesc(C.Structure):
+    _fields_ = [("n_lines", C.c_int32), ("n_samples", C.c_int32), ("fft_size", C.c_int32), ("stride", C.c_int32),
+                ("detection_threshold", C.c_double), ("mitigation_threshold", C.c_double)]
+
+
+class BurstDesc(C.Structure):
+    _fields_ = [("user_data", C.c_void_p), ("ud_offsets", C.c_void_p), ("ud_lengths", C.c_void_p),
+                ("num_quads", C.c_void_p), ("status", C.c_void_p), ("max_user_data_bytes", C.c_int32),
+                ("max_num_quads", C.c_int32), ("use_eccm", C.c_int32), ("reserved", C.c_int32),
+                ("range", RangeDesc), ("azimuth", AzimuthDesc), ("eccm", EccmDesc), ("eccm_flags", C.c_void_p),
+                ("out", C.c_void_p), ("ld_out", C.c_int64)]
+
+
+class PacketInfo(C.Structure):
+    _fields_ = [("primary", C.c_int64 * 8), ("secondary", C.c_int64 * 46), ("num_quads", C.c_int32),
+                ("baq_mode", C.c_int32), ("range_decimation", C.c_int32), ("rank", C.c_int32),
+                ("swath_number", C.c_int32), ("user_data_length", C.c_int32),
+                ("replica_chirp_length", C.c_int32), ("baq_block_length", C.c_int32), ("data_format", C.c_char),
+                ("reserved", C.c_char * 7), ("pri_us", C.c_double), ("pulse_length_us", C.c_double),
+                ("swl_us", C.c_double), ("swst_us", C.c_double), ("rx_gain_db", C.c_double),
+                ("txprr_mhz_us", C.c_double), ("txpsf_mhz", C.c_double), ("fs_mhz", C.c_double)]
+
+
+# (name, restype, argtypes) of every exported entry point
+_SIGS = [
+    ("s1g_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    ("s1g_destroy", C.c_int, [C.c_void_p]),
+    ("s1g_last_error", C.c_char_p, [C.c_void_p]),
+    ("s1g_set_stream", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("s1g_synchronize", C.c_int, [C.c_void_p]),
+    ("s1g_decode_fdbaq", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("s1g_fdbaq_symbol", C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("s1g_parse_packet_header", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PacketInfo)]),
+    ("s1g_slant_ranges", C.c_int, [C.POINTER(PacketInfo), C.c_int, C.c_void_p]),
+    ("s1g_timing_corrections", C.c_int, [C.POINTER(PacketInfo), C.c_void_p]),
+    ("s1g_range_compress", C.c_int, [C.c_void_p, C.POINTER(RangeDesc), C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                     C.c_int64]),
+    ("s1g_replica_chirp_length", C.c_int, [C.c_double, C.c_double]),
+    ("s1g_azimuth_fft", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_int]),
+    ("s1g_rcmc_matched_filter", C.c_int, [C.c_void_p, C.POINTER(AzimuthDesc), C.c_int, C.c_void_p, C.c_int64,
+                                          C.c_void_p, C.c_int64]),
+    ("s1g_eccm", C.c_int, [C.c_void_p, C.POINTER(EccmDesc), C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("s1g_focus_burst", C.c_int, [C.c_void_p, C.POINTER(BurstDesc), C.c_int]),
+    ("s1g_focus_batch", C.c_int, [C.c_void_p, C.POINTER(BurstDesc), C.c_int, C.c_int, C.c_int]),
+    ("s1g_af_forward", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                 C.c_void_p, C.POINTER(C.c_double)]),
+    ("s1g_af_adjoint", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                 C.POINTER(C.c_double), C.c_void_p]),
+    ("s1g_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
+    ("s1g_profile_read", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+]
+EXPORTED = [s[0] for s in _SIGS]
+STAGES = ["decode", "eccm", "range", "az_fft", "rcmc_mf", "az_ifft", "filter", "autofocus"]
+
+_LIB = None
+
+
+def lib():
+    """Load libs1gpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m sentinel1_decode_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class S1GError(RuntimeError):
+    pass
+
+
+# ------------------------------------------------------------ host helpers
+def parse_packet_header(packet: bytes) -> PacketInfo:
+    buf = np.frombuffer(bytes(packet), np.uint8)
+    info = PacketInfo()
+    rc = lib().s1g_parse_packet_header(buf.ctypes.data, buf.size, C.byref(info))
+    if rc:
+        raise S1GError(f"s1g_parse_packet_header failed ({rc})")
+    return info
+
+
+def slant_ranges(info: PacketInfo, num_ranges: int = 0) -> np.ndarray:
+    n = num_ranges if num_ranges > 0 else 2 * info.num_quads
+    out = np.zeros(n)
+    lib().s1g_slant_ranges(C.byref(info), n, out.ctypes.data)
+    return out
+
+
+def timing_corrections(info: PacketInfo) -> np.ndarray:
+    out = np.zeros(2 * info.num_quads)
+    lib().s1g_timing_corrections(C.byref(info), out.ctypes.data)
+    return out
+
+
+def fdbaq_symbol(brc: int, code10: int):
+    ln, m = C.c_int(0), C.c_int(0)
+    rc = lib().s1g_fdbaq_symbol(brc, code10, C.byref(ln), C.byref(m))
+    if rc:
+        raise S1GError("bad symbol query")
+    return ln.value, m.value
+
+
+# ------------------------------------------------------------- context
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One s1g context bound to a HIP device; calls follow torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        import torch
+        self.torch = torch
+        self.device = device
+        h = C.c_void_p()
+        rc = lib().s1g_create(C.byref(h), device)
+        if rc:
+            raise S1GError(f"s1g_create failed ({rc})")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().s1g_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bind_stream(self):
+        s = self.torch.cuda.current_stream(self.device).cuda_stream
+        lib().s1g_set_stream(self.h, C.c_void_p(s))
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().s1g_last_error(self.h)
+            raise S1GError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def profile(self, on=True):
+        self._check(lib().s1g_profile_enable(self.h, int(on)), "profile_enable")
+
+    def profile_read(self, reset=True):
+        ms = np.zeros(len(STAGES))
+        n = np.zeros(len(STAGES), np.int64)
+        self._check(lib().s1g_profile_read(self.h, ms.ctypes.data, n.ctypes.data, int(reset)), "profile_read")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(STAGES)}
+
+    def synchronize(self):
+        self._check(lib().s1g_synchronize(self.h), "synchronize")
+
+    # ---------------------------------------------------------------- decode
+    def decode(self, user_data, ud_offsets, ud_lengths, num_quads, max_bytes, max_quads, prec=S1G_F32,
+               out=None, status=None):
+        """Device tensors in, complex [n_packets, 2*max_quads] + int32 status out."""
+        torch = self.torch
+        n = ud_offsets.numel()
+        dt = torch.complex64 if prec == S1G_F32 else torch.complex128
+        if out is None:
+            out = torch.empty((n, 2 * max_quads), dtype=dt, device=user_data.device)
+        if status is None:
+            status = torch.empty(n, dtype=torch.int32, device=user_data.device)
+        self._bind_stream()
+        rc = lib().s1g_decode_fdbaq(self.h, _ptr(user_data), _ptr(ud_offsets), _ptr(ud_lengths), _ptr(num_quads), n,
+                                    int(max_bytes), int(max_quads), prec, _ptr(out), out.stride(0), _ptr(status))
+        self._check(rc, "decode")
+        return out, status
+
+    # ----------------------------------------------------------------- range
+    def range_compress(self, x, desc: RangeDesc, prec=None, out=None):
+        torch = self.torch
+        prec = _prec_of(x) if prec is None else prec
+        if out is None:
+            out = torch.empty_like(x)
+        self._bind_stream()
+        rc = lib().s1g_range_compress(self.h, C.byref(desc), prec, _ptr(x), x.stride(0), _ptr(out), out.stride(0))
+        self._check(rc, "range_compress")
+        return out
+
+    # --------------------------------------------------------------- azimuth
+    def azimuth_fft(self, x, inverse=False):
+        self._bind_stream()
+        rc = lib().s1g_azimuth_fft(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), x.stride(0), int(inverse))
+        self._check(rc, "azimuth_fft")
+        return x
+
+    def rcmc_matched_filter(self, x, desc: AzimuthDesc, out=None):
+        torch = self.torch
+        if out is None:
+            out = torch.empty_like(x)
+        self._bind_stream()
+        rc = lib().s1g_rcmc_matched_filter(self.h, C.byref(desc), _prec_of(x), _ptr(x), x.stride(0), _ptr(out),
+                                           out.stride(0))
+        self._check(rc, "rcmc_matched_filter")
+        return out
+
+    def eccm(self, x, desc: EccmDesc, flags=None):
+        self._bind_stream()
+        rc = lib().s1g_eccm(self.h, C.byref(desc), _prec_of(x), _ptr(x), x.stride(0), _ptr(flags))
+        self._check(rc, "eccm")
+        return x
+
+    # ----------------------------------------------------------------- chain
+    def focus_burst(self, desc: BurstDesc, prec):
+        self._bind_stream()
+        self._check(lib().s1g_focus_burst(self.h, C.byref(desc), prec), "focus_burst")
+
+    def focus_batch(self, descs, prec, graph=False):
+        arr = (BurstDesc * len(descs))(*descs)
+        self._bind_stream()
+        self._check(lib().s1g_focus_batch(self.h, arr, len(descs), prec, S1G_BATCH_GRAPH if graph else 0),
+                    "focus_batch")
+
+    # ------------------------------------------------------------- autofocus
+    def af_forward(self, rd, phase, img=None):
+        e = C.c_double(0)
+        self._bind_stream()
+        rc = lib().s1g_af_forward(self.h, _prec_of(rd), rd.shape[0], rd.shape[1], _ptr(rd), rd.stride(0),
+                                  _ptr(phase), _ptr(img), C.byref(e))
+        self._check(rc, "af_forward")
+        return e.value
+
+    def af_adjoint(self, rd, phase, grad):
+        e = C.c_double(0)
+        self._bind_stream()
+        rc = lib().s1g_af_adjoint(self.h, _prec_of(rd), rd.shape[0], rd.shape[1], _ptr(rd), rd.stride(0),
+                                  _ptr(phase), C.byref(e), _ptr(grad))
+        self._check(rc, "af_adjoint")
+        return e.value
+
+
+def _prec_of(x):
+    import torch
+    if x.dtype == torch.complex64:
+        return S1G_F32
+    if x.dtype == torch.complex128:
+        return S1G_F64
+    raise S1GError(f"unsupported dtype {x.dtype}")

@@ -1,0 +1,573 @@
+// capi.hip -- the C ABI (include/s1gpu.h): contexts, FFT plans, cached range
+// filters, workspace, and the chain entry points.  Host code only; kernels
+// live in the other .hip files.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "s1g_fdbaq.h"
+#include "s1g_internal.h"
+
+using namespace s1g;
+
+struct FilterKey {
+    double txpsf, txprr, txpl, fs;
+    int n, m, prec;
+    bool operator<(const FilterKey& o) const {
+        return std::tie(txpsf, txprr, txpl, fs, n, m, prec) < std::tie(o.txpsf, o.txprr, o.txpl, o.fs, o.n, o.m, o.prec);
+    }
+};
+
+struct s1g_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<int, DevFft> ffts;
+    std::map<FilterKey, void*> filters;
+    void* filter_scratch = nullptr;
+    size_t filter_scratch_bytes = 0;
+    void* ws = nullptr;  // chain workspace
+    size_t ws_bytes = 0;
+    AzColumn* cols = nullptr;
+    size_t cols_n = 0;
+    void* eccm_scratch = nullptr;
+    size_t eccm_scratch_bytes = 0;
+    double* af_scalars = nullptr;  // autofocus partial sums
+    size_t af_n = 0;
+    std::map<std::string, hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> graph_objs;
+    // stage timers
+    bool profiling = false;
+    bool capturing = false;
+    std::vector<hipEvent_t> event_pool;
+    struct Mark {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    double stage_ms[S1G_NSTAGES] = {0};
+    int64_t stage_n[S1G_NSTAGES] = {0};
+};
+
+// Brackets one stage's launches with events when profiling is on.
+struct StageTimer {
+    s1g_ctx* c;
+    int stage;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t take() {
+        if (!c->event_pool.empty()) {
+            hipEvent_t e = c->event_pool.back();
+            c->event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    StageTimer(s1g_ctx* ctx, int st) : c(ctx), stage(st) {
+        if (!c->profiling || c->capturing) return;
+        a = take();
+        b = take();
+        if (a) (void)hipEventRecord(a, c->stream);
+    }
+    ~StageTimer() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->marks.push_back({stage, a, b});
+    }
+};
+
+namespace {
+
+int fail(s1g_ctx* c, int code, const char* fmt, const char* detail = "") {
+    if (c) {
+        char buf[512];
+        snprintf(buf, sizeof buf, fmt, detail);
+        c->err = buf;
+    }
+    return code;
+}
+
+int hip_fail(s1g_ctx* c, hipError_t e, const char* where) {
+    if (c) c->err = std::string(where) + ": " + hipGetErrorString(e);
+    return S1G_EHIP;
+}
+
+#define HIPCHK(ctx, call)                                       \
+    do {                                                        \
+        hipError_t _e = (call);                                 \
+        if (_e != hipSuccess) return hip_fail(ctx, _e, #call);  \
+    } while (0)
+
+// Factor n over the LDS radices; biggest odd factors first (their pass then
+// runs with Ns = 1 and needs no twiddles), then 8s, a 4 or 2 for the rest.
+bool factorise(int n, FftPlan& p) {
+    static const int odd[] = {71, 67, 61, 59, 53, 47, 43, 41, 37, 31, 29, 23, 19, 17, 13, 11, 7, 5, 3};
+    p.n = n;
+    p.npass = 0;
+    int r = n;
+    std::vector<int> rad;
+    for (int f : odd)
+        while (r % f == 0) {
+            rad.push_back(f);
+            r /= f;
+        }
+    while (r % 8 == 0) {
+        rad.push_back(8);
+        r /= 8;
+    }
+    if (r % 4 == 0) {
+        rad.push_back(4);
+        r /= 4;
+    }
+    if (r % 2 == 0) {
+        rad.push_back(2);
+        r /= 2;
+    }
+    if (r != 1 || rad.size() > 24) return false;
+    for (size_t i = 0; i < rad.size(); ++i) p.radix[i] = rad[i];
+    p.npass = (int)rad.size();
+    return true;
+}
+
+int get_fft(s1g_ctx* c, int n, const DevFft** out) {
+    auto it = c->ffts.find(n);
+    if (it != c->ffts.end()) {
+        *out = &it->second;
+        return 0;
+    }
+    DevFft f;
+    if (!factorise(n, f.plan)) return fail(c, S1G_EUNSUPPORTED, "FFT length has a prime factor above 71%s");
+    std::vector<cpx<double>> t64(n);
+    std::vector<cpx<float>> t32(n);
+    for (int k = 0; k < n; ++k) {
+        // exact angle: reduce k/n to the first half turn before evaluating
+        long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+        double cr = (double)cosl(a), ci = (double)sinl(a);
+        t64[k].x = cr;
+        t64[k].y = ci;
+        t32[k].x = (float)cr;
+        t32[k].y = (float)ci;
+    }
+    // digit-reversed position of input i for DIT passes radix[0..S-1]
+    std::vector<int32_t> perm(n);
+    for (int i = 0; i < n; ++i) {
+        int64_t pos = 0, rem = i, m = n;
+        for (int s = f.plan.npass - 1; s >= 0; --s) {
+            const int R = f.plan.radix[s];
+            m /= R;
+            pos += (rem % R) * m;
+            rem /= R;
+        }
+        perm[i] = (int32_t)pos;
+    }
+    HIPCHK(c, hipMalloc(&f.tw64, sizeof(cpx<double>) * n));
+    HIPCHK(c, hipMalloc(&f.tw32, sizeof(cpx<float>) * n));
+    HIPCHK(c, hipMalloc(&f.perm, sizeof(int32_t) * n));
+    HIPCHK(c, hipMemcpy(f.tw64, t64.data(), sizeof(cpx<double>) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(f.tw32, t32.data(), sizeof(cpx<float>) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(f.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    auto ins = c->ffts.emplace(n, f);
+    *out = &ins.first->second;
+    return 0;
+}
+
+int ensure(s1g_ctx* c, void** p, size_t* have, size_t need) {
+    if (*have >= need) return 0;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    hipError_t e = hipMalloc(p, need);
+    if (e != hipSuccess) return fail(c, S1G_ENOMEM, "device allocation failed%s");
+    *have = need;
+    return 0;
+}
+
+size_t esize(int prec) { return prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>); }
+
+bool valid_prec(int prec) { return prec == S1G_F32 || prec == S1G_F64; }
+
+// Overlap-save FFT length: the smallest power of two holding a whole line
+// plus the chirp, else the largest one that fits 128 KB of LDS.
+int range_block_len(int prec, int n, int nc) {
+    const int mmax = prec == S1G_F32 ? 16384 : 8192;
+    int m = 1;
+    while (m < n + nc - 1 && m < mmax) m <<= 1;
+    if (m <= nc) return -1;
+    return m;
+}
+
+int get_filter(s1g_ctx* c, const s1g_range_desc& d, int prec, int nc, const DevFft* fm, void** h) {
+    FilterKey k{d.txpsf_mhz, d.txprr_mhz_us, d.txpl_us, d.fs_mhz, d.n_samples, fm->plan.n, prec};
+    auto it = c->filters.find(k);
+    if (it != c->filters.end()) {
+        *h = it->second;
+        return 0;
+    }
+    const int m = fm->plan.n;
+    int rc = ensure(c, &c->filter_scratch, &c->filter_scratch_bytes, sizeof(cpx<double>) * 3 * (size_t)m);
+    if (rc) return rc;
+    void* buf = nullptr;
+    HIPCHK(c, hipMalloc(&buf, esize(prec) * m));
+    rc = launch_range_filter(c->stream, *fm, d.txpsf_mhz, d.txprr_mhz_us, d.txpl_us, d.fs_mhz, d.n_samples, nc,
+                             c->filter_scratch, buf, prec);
+    if (rc) return hip_fail(c, (hipError_t)rc, "range filter");
+    c->filters[k] = buf;
+    *h = buf;
+    return 0;
+}
+
+int check_launch(s1g_ctx* c, int rc, const char* what) {
+    if (rc == -3) return fail(c, S1G_EUNSUPPORTED, "%s: shape does not fit the kernel", what);
+    if (rc == -1) return fail(c, S1G_EUNSUPPORTED, "%s: unsupported size", what);
+    if (rc) return hip_fail(c, (hipError_t)rc, what);
+    return 0;
+}
+
+int do_range(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int64_t ld_in, void* out,
+             int64_t ld_out) {
+    if (d.n_lines <= 0 || d.n_samples <= 0) return d.n_lines < 0 || d.n_samples < 0 ? fail(c, S1G_EARG, "bad range shape%s") : 0;
+    const int nc = (int)floor(d.fs_mhz * d.txpl_us);
+    if (nc < 2) return fail(c, S1G_EARG, "replica chirp shorter than 2 samples%s");
+    const int m = range_block_len(prec, d.n_samples, nc);
+    if (m < 0) return fail(c, S1G_EUNSUPPORTED, "replica chirp longer than the overlap-save block%s");
+    const DevFft* fm;
+    int rc = get_fft(c, m, &fm);
+    if (rc) return rc;
+    void* h;
+    {
+        StageTimer tm(c, S1G_STAGE_FILTER);
+        rc = get_filter(c, d, prec, nc, fm, &h);
+    }
+    if (rc) return rc;
+    StageTimer tm(c, S1G_STAGE_RANGE);
+    return check_launch(c, launch_range_os(c->stream, prec, *fm, in, ld_in, out, ld_out, d.n_lines, d.n_samples, nc, h),
+                        "range compression");
+}
+
+int do_azimuth_fft(s1g_ctx* c, int prec, int L, int N, void* data, int64_t ld, int inverse) {
+    if (L <= 0 || N <= 0) return 0;
+    const DevFft* fl;
+    int rc = get_fft(c, L, &fl);
+    if (rc) return rc;
+    StageTimer tm(c, inverse ? S1G_STAGE_AZ_IFFT : S1G_STAGE_AZ_FFT);
+    return check_launch(c, launch_azimuth_fft(c->stream, prec, *fl, data, ld, L, N, inverse), "azimuth fft");
+}
+
+int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int64_t ld_in, void* out,
+            int64_t ld_out) {
+    if (d.n_lines <= 1 || d.n_samples <= 0) return fail(c, S1G_EARG, "azimuth stage needs >= 2 lines%s");
+    if (!d.slant_ranges || !d.v_rel || !d.f_dc || !d.t_corr) return fail(c, S1G_EARG, "null geometry array%s");
+    int rc = ensure(c, (void**)&c->cols, &c->cols_n, sizeof(AzColumn) * (size_t)d.n_samples);
+    if (rc) return rc;
+    StageTimer tm(c, S1G_STAGE_RCMC_MF);
+    rc = launch_az_columns(c->stream, d, c->cols);
+    if (rc) return hip_fail(c, (hipError_t)rc, "azimuth columns");
+    return check_launch(c, launch_rcmc_mf(c->stream, prec, d, c->cols, in, ld_in, out, ld_out), "rcmc/matched filter");
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+extern "C" {
+
+int s1g_create(s1g_ctx** out, int device) {
+    if (!out) return S1G_EARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return S1G_EHIP;
+    if (device < 0 || device >= n) return S1G_EARG;
+    s1g_ctx* c = new (std::nothrow) s1g_ctx();
+    if (!c) return S1G_ENOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return S1G_EHIP;
+    }
+    c->stream = c->own;
+    *out = c;
+    return S1G_OK;
+}
+
+int s1g_destroy(s1g_ctx* c) {
+    if (!c) return S1G_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    for (auto& m : c->marks) {
+        hipEventDestroy(m.a);
+        hipEventDestroy(m.b);
+    }
+    for (auto e : c->event_pool) hipEventDestroy(e);
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    for (auto g : c->graph_objs) hipGraphDestroy(g);
+    for (auto& kv : c->ffts) {
+        hipFree(kv.second.tw64);
+        hipFree(kv.second.tw32);
+        hipFree(kv.second.perm);
+    }
+    for (auto& kv : c->filters) hipFree(kv.second);
+    hipFree(c->filter_scratch);
+    hipFree(c->ws);
+    hipFree(c->cols);
+    hipFree(c->eccm_scratch);
+    hipFree(c->af_scalars);
+    if (c->own) hipStreamDestroy(c->own);
+    delete c;
+    return S1G_OK;
+}
+
+const char* s1g_last_error(const s1g_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int s1g_set_stream(s1g_ctx* c, void* stream) {
+    if (!c) return S1G_EARG;
+    c->stream = stream ? (hipStream_t)stream : c->own;
+    return S1G_OK;
+}
+
+int s1g_synchronize(s1g_ctx* c) {
+    if (!c) return S1G_EARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return S1G_OK;
+}
+
+int s1g_profile_enable(s1g_ctx* c, int on) {
+    if (!c) return S1G_EARG;
+    c->profiling = on != 0;
+    return S1G_OK;
+}
+
+int s1g_profile_read(s1g_ctx* c, double* ms, int64_t* launches, int reset) {
+    if (!c) return S1G_EARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto& m : c->marks) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, m.a, m.b) == hipSuccess) {
+            c->stage_ms[m.stage] += t;
+            c->stage_n[m.stage] += 1;
+        }
+        c->event_pool.push_back(m.a);
+        c->event_pool.push_back(m.b);
+    }
+    c->marks.clear();
+    for (int i = 0; i < S1G_NSTAGES; ++i) {
+        if (ms) ms[i] = c->stage_ms[i];
+        if (launches) launches[i] = c->stage_n[i];
+        if (reset) {
+            c->stage_ms[i] = 0;
+            c->stage_n[i] = 0;
+        }
+    }
+    return S1G_OK;
+}
+
+int s1g_fdbaq_symbol(int brc, uint32_t code10, int* code_len, int* m_code) {
+    if (brc < 0 || brc > 4 || code10 > 1023u) return S1G_EARG;
+    if (code_len) *code_len = fdbaq_code_len(brc, code10);
+    if (m_code) *m_code = fdbaq_mcode(brc, code10);
+    return S1G_OK;
+}
+
+int s1g_replica_chirp_length(double fs_mhz, double txpl_us) { return (int)floor(fs_mhz * txpl_us); }
+
+// ---------------------------------------------------------------- headers
+// Field widths of the primary (48 bits) and secondary (496 bits) headers.
+static const int kPrimW[8] = {3, 1, 1, 7, 4, 2, 14, 16};
+static const int kSecW[46] = {32, 16, 32, 32, 8, 1, 3, 4, 32, 8, 16, 32, 32, 1, 2, 5, 8, 8, 8, 8, 1, 15, 1,
+                              15, 24, 3, 5, 24, 24, 24, 1, 3, 2, 2, 4, 2, 10, 2, 1, 5, 4, 3, 1, 8, 16, 8};
+static const double kFRef = 37.53472224;
+static const double kRangeDecimation[12] = {112.6041667, 100.0925926, 0.0,         83.41049387,
+                                            66.72839509, 56.30208336, 50.04629632, 25.02314816,
+                                            64.34523813, 46.91840280, 17.32371796, 54.59595962};
+
+static uint64_t be_field(const uint8_t* p, int64_t bit, int n) {
+    uint64_t v = 0;
+    for (int i = 0; i < n; ++i) {
+        const int64_t b = bit + i;
+        v = (v << 1) | ((p[b >> 3] >> (7 - (b & 7))) & 1u);
+    }
+    return v;
+}
+
+int s1g_parse_packet_header(const uint8_t* pk, int64_t nbytes, s1g_packet_info* info) {
+    if (!pk || !info || nbytes < 68) return S1G_EARG;
+    memset(info, 0, sizeof *info);
+    int64_t bit = 0;
+    for (int i = 0; i < 8; ++i) {
+        info->primary[i] = (int64_t)be_field(pk, bit, kPrimW[i]);
+        bit += kPrimW[i];
+    }
+    bit = 48;
+    for (int i = 0; i < 46; ++i) {
+        info->secondary[i] = (int64_t)be_field(pk, bit, kSecW[i]);
+        bit += kSecW[i];
+    }
+    const int64_t* s = info->secondary;
+    info->num_quads = (int32_t)s[44];
+    info->baq_mode = (int32_t)s[15];
+    info->range_decimation = (int32_t)s[18];
+    info->rank = (int32_t)s[26];
+    info->swath_number = (int32_t)s[43];
+    info->user_data_length = (int32_t)(info->primary[7] + 1 - 62);
+    info->baq_block_length = (int32_t)(8 * (s[16] + 1));
+    // data format (packet.cpp:265-274)
+    const int baq = info->baq_mode, test = (int)s[6];
+    if (baq == 0)
+        info->data_format = (test % 3 == 0) ? 'A' : 'B';
+    else if (baq >= 3 && baq <= 5)
+        info->data_format = 'C';
+    else if (baq >= 12 && baq <= 14)
+        info->data_format = 'D';
+    else
+        return S1G_EARG;
+    // physical values (packet.cpp:29-100, 277-281)
+    info->pri_us = (double)s[27] / kFRef;
+    info->pulse_length_us = (double)s[24] / kFRef;
+    info->swl_us = (double)s[29] / kFRef;
+    info->swst_us = (double)s[28] / kFRef;
+    info->rx_gain_db = -0.5 * (double)s[19];
+    info->txprr_mhz_us = (s[20] == 0 ? -1 : 1) * (double)s[21] * (pow(kFRef, 2) / 2097152);
+    const int sgn = s[22] == 0 ? -1 : 1;
+    info->txpsf_mhz = ((double)(sgn * (int)s[23]) * (kFRef / 16384)) + (info->txprr_mhz_us / (4 * kFRef));
+    info->fs_mhz = (s[18] >= 0 && s[18] < 12) ? kRangeDecimation[s[18]] : 0.0;
+    info->replica_chirp_length = (int32_t)floor(info->fs_mhz * info->pulse_length_us);
+    return S1G_OK;
+}
+
+int s1g_slant_ranges(const s1g_packet_info* info, int num_ranges, double* out) {
+    if (!info || !out) return S1G_EARG;
+    const double c = 299792458.0;
+    const double dts = (320 / (8 * kFRef)) * 1e-6;
+    if (num_ranges <= 0) num_ranges = 2 * info->num_quads;
+    const double nrs = 2 * info->num_quads;
+    const double fs = info->fs_mhz * 1e+6;
+    const double start = info->swst_us * 1e-6;
+    const double pri = info->pri_us * 1e-6;
+    const double rank = info->rank;
+    const double near = rank * pri + start + dts;
+    const double far = near + (nrs / fs);
+    const double a = near * c / 2, b = far * c / 2;
+    const double step = (b - a) / (num_ranges - 1);
+    for (int i = 0; i < num_ranges; ++i) out[i] = a + i * step;
+    return S1G_OK;
+}
+
+int s1g_timing_corrections(const s1g_packet_info* info, double* out) {
+    if (!info || !out) return S1G_EARG;
+    const int n = 2 * info->num_quads;
+    const double fs = info->fs_mhz * 1e+6;
+    const double swst = info->swst_us * 1e-6;
+    const double rank = info->rank;
+    const double pri = info->pri_us * 1e-6;
+    const double dt = 1 / fs;
+    for (int i = 0; i < n; ++i) out[i] = (swst + 1.439 * 1e-6) + ((swst + (rank * pri) - ((double)i * dt)) / 2);
+    return S1G_OK;
+}
+
+int s1g_decode_fdbaq(s1g_ctx* c, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens, const int32_t* nq,
+                     int n_packets, int max_bytes, int max_quads, int prec, void* out, int64_t ld_out,
+                     int32_t* status) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (n_packets < 0 || max_bytes < 0 || max_quads < 0) return fail(c, S1G_EARG, "negative size%s");
+    if (n_packets == 0) return S1G_OK;
+    if (!ud || !offs || !lens || !nq || !out || !status) return fail(c, S1G_EARG, "null pointer%s");
+    if (ld_out < 2 * (int64_t)max_quads) return fail(c, S1G_EARG, "ld_out < 2 * max_num_quads%s");
+    StageTimer tm(c, S1G_STAGE_DECODE);
+    int rc = launch_fdbaq_decode(c->stream, prec, ud, offs, lens, nq, n_packets, max_bytes, max_quads, out, ld_out,
+                                 status);
+    if (rc == -1) return fail(c, S1G_EUNSUPPORTED, "packet larger than the on-chip decode buffer%s");
+    if (rc) return hip_fail(c, (hipError_t)rc, "fdbaq decode");
+    return S1G_OK;
+}
+
+int s1g_range_compress(s1g_ctx* c, const s1g_range_desc* d, int prec, const void* in, int64_t ld_in, void* out,
+                       int64_t ld_out) {
+    if (!c || !d) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (!in || !out || in == out) return fail(c, S1G_EARG, "range compression is out-of-place%s");
+    if (ld_in < d->n_samples || ld_out < d->n_samples) return fail(c, S1G_EARG, "leading dimension too small%s");
+    return do_range(c, *d, prec, in, ld_in, out, ld_out);
+}
+
+int s1g_azimuth_fft(s1g_ctx* c, int prec, int L, int N, void* data, int64_t ld, int inverse) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (L < 0 || N < 0 || ld < N || (!data && L * N)) return fail(c, S1G_EARG, "bad azimuth fft arguments%s");
+    return do_azimuth_fft(c, prec, L, N, data, ld, inverse);
+}
+
+int s1g_rcmc_matched_filter(s1g_ctx* c, const s1g_azimuth_desc* d, int prec, const void* in, int64_t ld_in,
+                            void* out, int64_t ld_out) {
+    if (!c || !d) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (!in || !out || in == out) return fail(c, S1G_EARG, "rcmc/matched filter is out-of-place%s");
+    if (ld_in < d->n_samples || ld_out < d->n_samples) return fail(c, S1G_EARG, "leading dimension too small%s");
+    return do_rcmc(c, *d, prec, in, ld_in, out, ld_out);
+}
+
+int s1g_eccm(s1g_ctx* c, const s1g_eccm_desc* d, int prec, void* data, int64_t ld, uint8_t* flags) {
+    if (!c || !d) return S1G_EARG;
+    return fail(c, S1G_EUNSUPPORTED, "ECCM is not available in this build%s");
+}
+
+static int burst_chain(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
+    const int L = b.range.n_lines, N = b.range.n_samples;
+    if (L <= 1 || N <= 0) return fail(c, S1G_EARG, "burst needs >= 2 lines%s");
+    if (b.azimuth.n_lines != L || b.azimuth.n_samples != N) return fail(c, S1G_EARG, "descriptor shapes disagree%s");
+    if (b.ld_out < N || !b.out) return fail(c, S1G_EARG, "bad output buffer%s");
+    int rc = ensure(c, &c->ws, &c->ws_bytes, esize(prec) * (size_t)L * N);
+    if (rc) return rc;
+    rc = s1g_decode_fdbaq(c, b.user_data, b.ud_offsets, b.ud_lengths, b.num_quads, L, b.max_user_data_bytes,
+                          b.max_num_quads, prec, b.out, b.ld_out, b.status);
+    if (rc) return rc;
+    if (b.use_eccm) {
+        rc = s1g_eccm(c, &b.eccm, prec, b.out, b.ld_out, b.eccm_flags);
+        if (rc) return rc;
+    }
+    rc = do_range(c, b.range, prec, b.out, b.ld_out, c->ws, N);
+    if (rc) return rc;
+    rc = do_azimuth_fft(c, prec, L, N, c->ws, N, 0);
+    if (rc) return rc;
+    rc = do_rcmc(c, b.azimuth, prec, c->ws, N, b.out, b.ld_out);
+    if (rc) return rc;
+    return do_azimuth_fft(c, prec, L, N, b.out, b.ld_out, 1);
+}
+
+int s1g_focus_burst(s1g_ctx* c, const s1g_burst_desc* b, int prec) {
+    if (!c || !b) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    return burst_chain(c, *b, prec);
+}
+
+int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, int flags) {
+    if (!c || (!descs && n)) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    for (int i = 0; i < n; ++i) {
+        int rc = burst_chain(c, descs[i], prec);
+        if (rc) return rc;
+    }
+    (void)flags;
+    return S1G_OK;
+}
+
+int s1g_af_forward(s1g_ctx* c, int prec, int L, int N, const void* rd, int64_t ld, const double* phase, void* img,
+                   double* entropy) {
+    if (!c) return S1G_EARG;
+    return fail(c, S1G_EUNSUPPORTED, "autofocus is not available in this build%s");
+}
+
+int s1g_af_adjoint(s1g_ctx* c, int prec, int L, int N, const void* rd, int64_t ld, const double* phase,
+                   double* entropy, double* grad) {
+    if (!c) return S1G_EARG;
+    return fail(c, S1G_EUNSUPPORTED, "autofocus is not available in this build%s");
+}
+
+}  // extern "C"

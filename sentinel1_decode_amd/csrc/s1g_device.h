@@ -1,0 +1,341 @@
+// s1g_device.h -- device-side building blocks shared by the gfx950 kernels:
+// complex arithmetic, compile-time trigonometry, radix-R DFT butterflies and
+// the in-place mixed-radix DIT / DIF passes that run on LDS-resident data.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace s1g {
+
+// ---------------------------------------------------------------- complex
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> mk(T a, T b) {
+    cpx<T> r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+    return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> scl(cpx<T> a, T s) { return mk(a.x * s, a.y * s); }
+
+// ------------------------------------------------- compile-time trigonometry
+// cos(2 pi num / den) and sin(2 pi num / den) for integers, evaluated at
+// compile time: exact octant reduction on the rational angle, then Taylor
+// series on |x| <= pi/4 (26 terms, beyond double precision).
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int k = 1; k < 26; ++k) {
+        term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 26; ++k) {
+        term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+// angle = num/den of a full turn, 0 <= num < den
+constexpr double turn_cos(long num, long den);
+constexpr double turn_sin(long num, long den) {
+    num %= den;
+    if (num < 0) num += den;
+    if (2 * num > den) return -turn_sin(den - num, den);      // sin(2pi - a) = -sin a
+    if (4 * num > den) return turn_sin(den - 2 * num, 2 * den);  // sin(pi - a) = sin a
+    if (8 * num > den) return turn_cos(den - 4 * num, 4 * den);  // sin(pi/2 - a) = cos a
+    return taylor_sin(kTwoPi * (double)num / (double)den);
+}
+constexpr double turn_cos(long num, long den) {
+    num %= den;
+    if (num < 0) num += den;
+    if (2 * num > den) return turn_cos(den - num, den);
+    if (4 * num > den) return -turn_cos(den - 2 * num, 2 * den);
+    if (8 * num > den) return turn_sin(den - 4 * num, 4 * den);
+    return taylor_cos(kTwoPi * (double)num / (double)den);
+}
+
+// ------------------------------------------------------------ butterflies
+// In-register DFT of length R.  INV = false: X_k = sum x_m e^{-2 pi i mk/R};
+// INV = true uses e^{+...}.  a[] is overwritten with the result.
+template <typename T, int R, bool INV>
+struct Dft;
+
+template <typename T, bool INV>
+struct Dft<T, 2, INV> {
+    static __device__ __forceinline__ void run(cpx<T>* a) {
+        cpx<T> t = a[0];
+        a[0] = t + a[1];
+        a[1] = t - a[1];
+    }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 4, INV> {
+    static __device__ __forceinline__ void run(cpx<T>* a) {
+        cpx<T> s0 = a[0] + a[2], d0 = a[0] - a[2];
+        cpx<T> s1 = a[1] + a[3], d1 = a[1] - a[3];
+        // forward: X1 = d0 - i d1, X3 = d0 + i d1
+        cpx<T> jd1 = INV ? mk(-d1.y, d1.x) : mk(d1.y, -d1.x);
+        a[0] = s0 + s1;
+        a[2] = s0 - s1;
+        a[1] = d0 + jd1;
+        a[3] = d0 - jd1;
+    }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 8, INV> {
+    static __device__ __forceinline__ void run(cpx<T>* a) {
+        const T h = (T)0.70710678118654752440084436210485;
+        cpx<T> e[4] = {a[0], a[2], a[4], a[6]};
+        cpx<T> o[4] = {a[1], a[3], a[5], a[7]};
+        Dft<T, 4, INV>::run(e);
+        Dft<T, 4, INV>::run(o);
+        // twiddles w8^k for k = 1, 2, 3
+        cpx<T> o1 = INV ? mk(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                        : mk(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+        cpx<T> o2 = INV ? mk(-o[2].y, o[2].x) : mk(o[2].y, -o[2].x);
+        cpx<T> o3 = INV ? mk(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                        : mk(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+        a[0] = e[0] + o[0];
+        a[4] = e[0] - o[0];
+        a[1] = e[1] + o1;
+        a[5] = e[1] - o1;
+        a[2] = e[2] + o2;
+        a[6] = e[2] - o2;
+        a[3] = e[3] + o3;
+        a[7] = e[3] - o3;
+    }
+};
+
+// Odd R: pair outputs k and R-k,
+//   X_k     = x0 + sum_m (x_m + x_{R-m}) cos(2 pi mk/R) -/+ i sum_m (x_m - x_{R-m}) sin(2 pi mk/R)
+//   X_{R-k} = x0 + ...                                  +/- i ...
+// The (k, m) loops are template recursions so every cos/sin is a
+// compile-time constant.
+template <typename T, int R, int K, int M>
+__device__ __forceinline__ void odd_acc(const cpx<T>* s, const cpx<T>* d, T& ar, T& ai, T& br, T& bi) {
+    if constexpr (M <= (R - 1) / 2) {
+        constexpr T c = (T)turn_cos((long)M * K, R);
+        constexpr T sn = (T)turn_sin((long)M * K, R);
+        ar += s[M - 1].x * c;
+        ai += s[M - 1].y * c;
+        br += d[M - 1].x * sn;
+        bi += d[M - 1].y * sn;
+        odd_acc<T, R, K, M + 1>(s, d, ar, ai, br, bi);
+    }
+}
+
+template <typename T, int R, bool INV, int K>
+__device__ __forceinline__ void odd_out(cpx<T>* a, const cpx<T>* s, const cpx<T>* d, cpx<T> x0) {
+    if constexpr (K <= (R - 1) / 2) {
+        T ar = x0.x, ai = x0.y, br = 0, bi = 0;
+        odd_acc<T, R, K, 1>(s, d, ar, ai, br, bi);
+        // forward: X_k = A - i B ; inverse: X_k = A + i B
+        if (INV) {
+            a[K] = mk(ar - bi, ai + br);
+            a[R - K] = mk(ar + bi, ai - br);
+        } else {
+            a[K] = mk(ar + bi, ai - br);
+            a[R - K] = mk(ar - bi, ai + br);
+        }
+        odd_out<T, R, INV, K + 1>(a, s, d, x0);
+    }
+}
+
+template <typename T, int R, bool INV>
+struct Dft {
+    static_assert(R % 2 == 1, "generic butterfly handles odd lengths");
+    static constexpr int H = (R - 1) / 2;
+    static __device__ __forceinline__ void run(cpx<T>* a) {
+        cpx<T> s[H], d[H];
+        const cpx<T> x0 = a[0];
+        cpx<T> sum = a[0];
+#pragma unroll
+        for (int m = 1; m <= H; ++m) {
+            s[m - 1] = a[m] + a[R - m];
+            d[m - 1] = a[m] - a[R - m];
+            sum = sum + s[m - 1];
+        }
+        a[0] = sum;
+        odd_out<T, R, INV, 1>(a, s, d, x0);
+    }
+};
+
+// Radices the LDS passes support; the host planner factors lengths over them.
+#define S1G_RADIX_LIST(X) X(2) X(3) X(4) X(5) X(7) X(8) X(11) X(13) X(17) X(19) X(23) X(29) X(31) X(37) X(41) X(43) X(47) X(53) X(59) X(61) X(67) X(71)
+
+// ----------------------------------------------------- LDS pass primitives
+// Data layout in LDS: element e of vector v lives at buf[v * vstride + e * estride].
+// Twiddle table tw[k] = e^{-2 pi i k / n} (k < n) in global memory.
+//
+// In-place decimation-in-time pass with radix R after Ns = product of the
+// radices already applied: butterfly (g, j) combines elements
+// g*Ns*R + j + r*Ns, r < R, multiplying input r by w_{Ns R}^{j r} first.
+// The input of the first pass must be in mixed-radix digit-reversed order.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dit_pass(cpx<T>* buf, int n, int Ns, int nvec, int vstride, int estride,
+                                         const cpx<T>* __restrict__ tw, int tid, int nthr) {
+    const int nbf = n / R;
+    const int total = nbf * nvec;
+    const int tstep = n / (Ns * R);
+    for (int idx = tid; idx < total; idx += nthr) {
+        const int v = idx % nvec;
+        const int bf = idx / nvec;
+        const int g = bf / Ns, j = bf - g * Ns;
+        const int base = (g * Ns * R + j) * estride + v * vstride;
+        cpx<T> a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = buf[base + r * Ns * estride];
+        if (j) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cpx<T> w = tw[j * r * tstep];
+                a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+            }
+        }
+        Dft<T, R, INV>::run(a);
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+    }
+}
+
+// In-place decimation-in-frequency pass (transpose of dit_pass): butterfly
+// first, then multiply output r by w_{Ns R}^{j r}.  Running the DIT passes'
+// geometry in reverse order on natural-order input leaves the transform in
+// digit-reversed order.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dif_pass(cpx<T>* buf, int n, int Ns, int nvec, int vstride, int estride,
+                                         const cpx<T>* __restrict__ tw, int tid, int nthr) {
+    const int nbf = n / R;
+    const int total = nbf * nvec;
+    const int tstep = n / (Ns * R);
+    for (int idx = tid; idx < total; idx += nthr) {
+        const int v = idx % nvec;
+        const int bf = idx / nvec;
+        const int g = bf / Ns, j = bf - g * Ns;
+        const int base = (g * Ns * R + j) * estride + v * vstride;
+        cpx<T> a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = buf[base + r * Ns * estride];
+        Dft<T, R, INV>::run(a);
+        if (j) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cpx<T> w = tw[j * r * tstep];
+                a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+    }
+}
+
+// A factorisation of n into LDS radices (first pass first).
+struct FftPlan {
+    int n;
+    int npass;
+    int radix[24];
+};
+
+#define S1G_POW2_LIST(X) X(2) X(4) X(8)
+
+// POW2 = true restricts the dispatch to radices 2, 4, 8 (power-of-two plans).
+template <typename T, bool INV, bool POW2>
+__device__ __forceinline__ void dispatch_dit(int R, cpx<T>* buf, int n, int Ns, int nvec, int vstride,
+                                             int estride, const cpx<T>* tw, int tid, int nthr) {
+#define S1G_CASE(r)                                                                  \
+    case r:                                                                          \
+        dit_pass<T, r, INV>(buf, n, Ns, nvec, vstride, estride, tw, tid, nthr);      \
+        break;
+    if constexpr (POW2) {
+        switch (R) { S1G_POW2_LIST(S1G_CASE) default: break; }
+    } else {
+        switch (R) { S1G_RADIX_LIST(S1G_CASE) default: break; }
+    }
+#undef S1G_CASE
+}
+
+template <typename T, bool INV, bool POW2>
+__device__ __forceinline__ void dispatch_dif(int R, cpx<T>* buf, int n, int Ns, int nvec, int vstride,
+                                             int estride, const cpx<T>* tw, int tid, int nthr) {
+#define S1G_CASE(r)                                                                  \
+    case r:                                                                          \
+        dif_pass<T, r, INV>(buf, n, Ns, nvec, vstride, estride, tw, tid, nthr);      \
+        break;
+    if constexpr (POW2) {
+        switch (R) { S1G_POW2_LIST(S1G_CASE) default: break; }
+    } else {
+        switch (R) { S1G_RADIX_LIST(S1G_CASE) default: break; }
+    }
+#undef S1G_CASE
+}
+
+// Full forward/inverse transform of nvec LDS-resident vectors whose input
+// already sits in digit-reversed order; output in natural order.
+template <typename T, bool INV, bool POW2 = false>
+__device__ __forceinline__ void fft_dit_lds(cpx<T>* buf, const FftPlan& p, int nvec, int vstride, int estride,
+                                            const cpx<T>* tw, int tid, int nthr) {
+    int Ns = 1;
+    for (int s = 0; s < p.npass; ++s) {
+        const int R = p.radix[s];
+        dispatch_dit<T, INV, POW2>(R, buf, p.n, Ns, nvec, vstride, estride, tw, tid, nthr);
+        Ns *= R;
+        __syncthreads();
+    }
+}
+
+// Natural-order input; output in digit-reversed order.
+template <typename T, bool INV, bool POW2 = false>
+__device__ __forceinline__ void fft_dif_lds(cpx<T>* buf, const FftPlan& p, int nvec, int vstride, int estride,
+                                            const cpx<T>* tw, int tid, int nthr) {
+    int Ns = p.n;
+    for (int s = p.npass - 1; s >= 0; --s) {
+        const int R = p.radix[s];
+        Ns /= R;
+        dispatch_dif<T, INV, POW2>(R, buf, p.n, Ns, nvec, vstride, estride, tw, tid, nthr);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------- FP64 phase helpers
+// Reduction of a large FP64 phase to [-pi, pi] (Cody & Waite 1980 additive
+// reduction): 2 pi = C1 + C2 + C3 with C1, C2 holding 27 significant bits, so
+// k*C1 and k*C2 are exact for |k| < 2^26 (|phase| < 4e8 rad).
+__device__ __forceinline__ double reduce_2pi(double ph) {
+#pragma clang fp contract(off)
+    const double inv2pi = 0.15915494309189535;
+    const double C1 = 6.283185303211212;       // 27 significant bits
+    const double C2 = 3.968374295837407e-09;   // next 27 bits
+    const double C3 = 2.2884754904439327e-17;  // remainder
+    double k = rint(ph * inv2pi);
+    double r = ph - k * C1;
+    r = r - k * C2;
+    r = r - k * C3;
+    return r;
+}
+
+}  // namespace s1g

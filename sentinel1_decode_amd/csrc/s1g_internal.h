@@ -1,0 +1,72 @@
+// s1g_internal.h -- declarations shared between the kernel files and the
+// C-ABI implementation (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/s1gpu.h"
+#include "s1g_device.h"
+
+namespace s1g {
+
+// Device-resident FFT plan of one length: radix list plus twiddle tables
+// (FP64 and FP32, e^{-2 pi i k / n}) and the mixed-radix digit-reversal map
+// perm[i] = position of input element i for the in-place DIT passes.
+struct DevFft {
+    FftPlan plan;
+    cpx<double>* tw64 = nullptr;
+    cpx<float>* tw32 = nullptr;
+    int32_t* perm = nullptr;
+};
+
+template <typename T>
+__host__ __device__ inline const cpx<T>* tw_of(const DevFft& f);
+template <>
+__host__ __device__ inline const cpx<double>* tw_of<double>(const DevFft& f) { return f.tw64; }
+template <>
+__host__ __device__ inline const cpx<float>* tw_of<float>(const DevFft& f) { return f.tw32; }
+
+// Per-column constants of the range-Doppler stage (FP32 chain).
+struct AzColumn {
+    double a;     // lambda^2 / (4 v_rel^2)
+    double b;     // 4 pi R_j / lambda
+    double c;     // 2 pi t_corr_j
+    double rdr;   // R_j / range_spacing
+    double fdc;   // Doppler centroid
+    double amp;   // 1 / sqrt((R_ref / R_j)^3)
+};
+
+int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
+                        const int32_t* nqs, int n_packets, int max_bytes, int max_quads, void* out, int64_t ld_out,
+                        int32_t* status);
+
+// Out-of-place Stockham FFT of `batch` vectors of length n (vector b at
+// in + b*dist) in global memory, one launch per pass; the result ends in
+// `out` (`tmp` is scratch of the same size).
+int launch_fft_global(hipStream_t stream, int prec, const DevFft& f, const void* in, void* out, void* tmp, int batch,
+                      int64_t dist, int inverse);
+
+// Range compression: build the overlap-save filter, then filter the lines.
+int launch_range_filter(hipStream_t stream, const DevFft& fm, double txpsf, double txprr, double txpl, double fs_mhz,
+                        int n_samples, int nc, void* scratch, void* h_out, int prec);
+int launch_range_os(hipStream_t stream, int prec, const DevFft& fm, const void* in, int64_t ld_in, void* out,
+                    int64_t ld_out, int n_lines, int n_samples, int nc, const void* h);
+
+// Azimuth stages.
+int launch_azimuth_fft(hipStream_t stream, int prec, const DevFft& fl, void* data, int64_t ld, int n_lines,
+                       int n_samples, int inverse);
+int launch_az_columns(hipStream_t stream, const s1g_azimuth_desc& d, AzColumn* cols);
+int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
+                   int64_t ld_in, void* out, int64_t ld_out);
+
+// ECCM.
+int launch_eccm(hipStream_t stream, int prec, const s1g_eccm_desc& d, const DevFft* fdet, int det_split,
+                const DevFft& fwin, void* data, int64_t ld, uint8_t* flags, void* scratch);
+
+// Autofocus.
+int launch_af(hipStream_t stream, int prec, int n_lines, int n_samples, const void* rd, int64_t ld,
+              const double* phase, const DevFft& fl, void* img, double* partial, int want_grad, double* grad,
+              double* scalars);
+
+}  // namespace s1g
