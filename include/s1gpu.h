@@ -58,8 +58,8 @@ typedef struct s1g_ctx s1g_ctx;
 int s1g_create(s1g_ctx** ctx, int device);
 int s1g_destroy(s1g_ctx* ctx);
 const char* s1g_last_error(const s1g_ctx* ctx);
-/* Run subsequent work on `stream` (a hipStream_t); NULL restores the
- * context's own stream. */
+/* Run subsequent work on `stream` (a hipStream_t; NULL is the null stream).
+ * Until this is called a context uses a stream of its own. */
 int s1g_set_stream(s1g_ctx* ctx, void* stream);
 int s1g_synchronize(s1g_ctx* ctx);
 

@@ -23,12 +23,22 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
 
 
-def build_gpu(verbose=False, jobs=8):
+def _headers():
+    return glob.glob(os.path.join(CSRC, "*.h")) + [os.path.join(HERE, "..", "include", "s1gpu.h")]
+
+
+def build_gpu(verbose=False, jobs=8, force=True):
+    """Compile every .hip source (force=False skips objects newer than their
+    source and all headers) and link libs1gpu.so."""
     os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
     objs, procs = [], []
+    newest_header = max(os.path.getmtime(h) for h in _headers())
     for src in sources():
         obj = os.path.join(HERE, "_obj", os.path.basename(src) + ".o")
         objs.append(obj)
+        if (not force and os.path.exists(obj) and os.path.getmtime(obj) > os.path.getmtime(src)
+                and os.path.getmtime(obj) > newest_header):
+            continue
         cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(HERE, "..", "include"), "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -58,7 +68,7 @@ def build_synth():
 
 
 def main():
-    build_gpu(verbose="-v" in sys.argv)
+    build_gpu(verbose="-v" in sys.argv, force="--force" in sys.argv)
     build_synth()
     print(LIB)
 

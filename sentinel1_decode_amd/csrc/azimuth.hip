@@ -143,17 +143,19 @@ struct SincWeights;
 
 template <>
 struct SincWeights<float> {
+    // sin(pi (fr - k)) = (-1)^k sinpi(fr); sinpi/cospi reduce their argument
+    // exactly, so the weights keep full relative accuracy for fr near 0 or 1.
     static __device__ __forceinline__ void eval(float fr, float* w) {
         const float pi = 3.14159265358979f;
         float S, C, s8, c8;
-        sincosf(pi * fr, &S, &C);
-        sincosf(pi * fr * 0.125f, &s8, &c8);
+        sincospif(fr, &S, &C);
+        sincospif(fr * 0.125f, &s8, &c8);
 #pragma unroll
         for (int k = -kTaps; k <= kTaps; ++k) {
             const float x = fr - (float)k;
             const float win = 0.5f * (1.0f + (c8 * (float)kCosK8[k + kTaps] + s8 * (float)kSinK8[k + kTaps]));
             const float sgn = (k & 1) ? -1.0f : 1.0f;
-            const float sinc = (x == 0.0f) ? 1.0f : (sgn * S) * __frcp_rn(pi * x);
+            const float sinc = (x == 0.0f) ? 1.0f : (sgn * S) / (pi * x);
             w[k + kTaps] = sinc * win;
         }
     }
@@ -161,18 +163,22 @@ struct SincWeights<float> {
 
 template <>
 struct SincWeights<double> {
+    // The reference evaluates sin(PI (fr - k)) per tap; the two nearest taps
+    // take exactly its arguments: PI fr (k = 0) and -PI (1 - fr) (k = 1).
+    // Even taps reuse sin(PI fr), odd taps -sin(PI (1 - fr)).
     static __device__ __forceinline__ void eval(double fr, double* w) {
 #pragma clang fp contract(off)
         const double PI = 3.14159265358979;
-        double S, C, s8, c8;
-        sincos(PI * fr, &S, &C);
+        const double Se = sin(PI * fr);
+        const double So = sin(PI * (1.0 - fr));
+        double s8, c8;
         sincos(PI * fr / kTaps, &s8, &c8);
 #pragma unroll
         for (int k = -kTaps; k <= kTaps; ++k) {
             const double x = fr - (double)k;
             const double win = 0.5 * (1 + (c8 * kCosK8[k + kTaps] + s8 * kSinK8[k + kTaps]));
-            const double sgn = (k & 1) ? -1.0 : 1.0;
-            const double sinc = (x == 0.0) ? 1.0 : (sgn * S) / (PI * x);
+            const double num = (k & 1) ? -So : Se;
+            const double sinc = (x == 0.0) ? 1.0 : num / (PI * x);
             w[k + kTaps] = sinc * win;
         }
     }

@@ -145,7 +145,12 @@ int get_fft(s1g_ctx* c, int n, const DevFft** out) {
         return 0;
     }
     DevFft f;
-    if (!factorise(n, f.plan)) return fail(c, S1G_EUNSUPPORTED, "FFT length has a prime factor above 71%s");
+    if (!factorise(n, f.plan)) {
+        // twiddle table only (the length has a prime factor above 71): usable
+        // as a table of e^{-2 pi i k / n}, not as an LDS transform plan
+        f.plan.n = n;
+        f.plan.npass = -1;
+    }
     std::vector<cpx<double>> t64(n);
     std::vector<cpx<float>> t32(n);
     for (int k = 0; k < n; ++k) {
@@ -160,7 +165,8 @@ int get_fft(s1g_ctx* c, int n, const DevFft** out) {
     // digit-reversed position of input i for DIT passes radix[0..S-1]
     std::vector<int32_t> perm(n);
     for (int i = 0; i < n; ++i) {
-        int64_t pos = 0, rem = i, m = n;
+        int64_t pos = i, rem = i, m = n;
+        if (f.plan.npass > 0) pos = 0;
         for (int s = f.plan.npass - 1; s >= 0; --s) {
             const int R = f.plan.radix[s];
             m /= R;
@@ -205,6 +211,13 @@ int range_block_len(int prec, int n, int nc) {
     return m;
 }
 
+int get_plan(s1g_ctx* c, int n, const DevFft** out) {
+    int rc = get_fft(c, n, out);
+    if (rc) return rc;
+    if ((*out)->plan.npass < 0) return fail(c, S1G_EUNSUPPORTED, "FFT length has a prime factor above 71%s");
+    return 0;
+}
+
 int get_filter(s1g_ctx* c, const s1g_range_desc& d, int prec, int nc, const DevFft* fm, void** h) {
     FilterKey k{d.txpsf_mhz, d.txprr_mhz_us, d.txpl_us, d.fs_mhz, d.n_samples, fm->plan.n, prec};
     auto it = c->filters.find(k);
@@ -240,7 +253,7 @@ int do_range(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int6
     const int m = range_block_len(prec, d.n_samples, nc);
     if (m < 0) return fail(c, S1G_EUNSUPPORTED, "replica chirp longer than the overlap-save block%s");
     const DevFft* fm;
-    int rc = get_fft(c, m, &fm);
+    int rc = get_plan(c, m, &fm);
     if (rc) return rc;
     void* h;
     {
@@ -256,7 +269,7 @@ int do_range(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int6
 int do_azimuth_fft(s1g_ctx* c, int prec, int L, int N, void* data, int64_t ld, int inverse) {
     if (L <= 0 || N <= 0) return 0;
     const DevFft* fl;
-    int rc = get_fft(c, L, &fl);
+    int rc = get_plan(c, L, &fl);
     if (rc) return rc;
     StageTimer tm(c, inverse ? S1G_STAGE_AZ_IFFT : S1G_STAGE_AZ_FFT);
     return check_launch(c, launch_azimuth_fft(c->stream, prec, *fl, data, ld, L, N, inverse), "azimuth fft");
@@ -272,6 +285,39 @@ int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int
     rc = launch_az_columns(c->stream, d, c->cols);
     if (rc) return hip_fail(c, (hipError_t)rc, "azimuth columns");
     return check_launch(c, launch_rcmc_mf(c->stream, prec, d, c->cols, in, ld_in, out, ld_out), "rcmc/matched filter");
+}
+
+int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_t ld_in, void* out, int64_t ld_out,
+            uint8_t* flags) {
+    const int L = d.n_lines, N = d.n_samples;
+    if (L <= 0 || N <= 0) return L < 0 || N < 0 ? fail(c, S1G_EARG, "bad ECCM shape%s") : 0;
+    const int fsz = d.fft_size;
+    if (fsz < 4 || fsz > 1024 || (fsz & (fsz - 1)) || d.stride < 1 || d.stride > fsz)
+        return fail(c, S1G_EUNSUPPORTED, "ECCM window must be a power of two in [4, 1024] with 1 <= stride <= size%s");
+    if ((fsz + d.stride - 1) / d.stride > 8) return fail(c, S1G_EUNSUPPORTED, "ECCM window overlap too deep%s");
+    // detection split r: N/r must fit 128 KB of LDS and factor over the radices
+    int r = 0;
+    static const int cand[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
+    for (int k : cand) {
+        if (N % k) continue;
+        FftPlan p;
+        if ((size_t)(N / k) * esize(prec) <= 128 * 1024 && factorise(N / k, p)) {
+            r = k;
+            break;
+        }
+    }
+    if (!r) return fail(c, S1G_EUNSUPPORTED, "ECCM detection length does not split over the LDS radices%s");
+    const DevFft *fN, *fsub, *fw;
+    int rc = get_fft(c, N, &fN);
+    if (!rc) rc = get_plan(c, N / r, &fsub);
+    if (!rc) rc = get_plan(c, fsz, &fw);
+    if (rc) return rc;
+    rc = ensure(c, &c->eccm_scratch, &c->eccm_scratch_bytes, sizeof(double) * (size_t)L * r);
+    if (rc) return rc;
+    StageTimer tm(c, S1G_STAGE_ECCM);
+    return check_launch(c, launch_eccm(c->stream, prec, d, *fN, *fsub, r, *fw, in, ld_in, out, ld_out, flags,
+                                       c->eccm_scratch),
+                        "eccm");
 }
 
 }  // namespace
@@ -328,7 +374,7 @@ const char* s1g_last_error(const s1g_ctx* c) { return c ? c->err.c_str() : "null
 
 int s1g_set_stream(s1g_ctx* c, void* stream) {
     if (!c) return S1G_EARG;
-    c->stream = stream ? (hipStream_t)stream : c->own;
+    c->stream = (hipStream_t)stream;  // NULL: the null stream
     return S1G_OK;
 }
 
@@ -515,7 +561,17 @@ int s1g_rcmc_matched_filter(s1g_ctx* c, const s1g_azimuth_desc* d, int prec, con
 
 int s1g_eccm(s1g_ctx* c, const s1g_eccm_desc* d, int prec, void* data, int64_t ld, uint8_t* flags) {
     if (!c || !d) return S1G_EARG;
-    return fail(c, S1G_EUNSUPPORTED, "ECCM is not available in this build%s");
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (!data || ld < d->n_samples) return fail(c, S1G_EARG, "bad ECCM buffer%s");
+    if (d->n_lines <= 0 || d->n_samples <= 0) return S1G_OK;
+    const size_t row = esize(prec) * (size_t)d->n_samples;
+    int rc = ensure(c, &c->ws, &c->ws_bytes, row * d->n_lines);
+    if (rc) return rc;
+    rc = do_eccm(c, *d, prec, data, ld, c->ws, d->n_samples, flags);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy2DAsync(data, esize(prec) * ld, c->ws, row, row, d->n_lines, hipMemcpyDeviceToDevice,
+                               c->stream));
+    return S1G_OK;
 }
 
 static int burst_chain(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
@@ -525,13 +581,18 @@ static int burst_chain(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
     if (b.ld_out < N || !b.out) return fail(c, S1G_EARG, "bad output buffer%s");
     int rc = ensure(c, &c->ws, &c->ws_bytes, esize(prec) * (size_t)L * N);
     if (rc) return rc;
-    rc = s1g_decode_fdbaq(c, b.user_data, b.ud_offsets, b.ud_lengths, b.num_quads, L, b.max_user_data_bytes,
-                          b.max_num_quads, prec, b.out, b.ld_out, b.status);
-    if (rc) return rc;
     if (b.use_eccm) {
-        rc = s1g_eccm(c, &b.eccm, prec, b.out, b.ld_out, b.eccm_flags);
+        // decode -> workspace -> ECCM -> out (ECCM is out of place)
+        if (b.eccm.n_lines != L || b.eccm.n_samples != N) return fail(c, S1G_EARG, "ECCM shape disagrees%s");
+        rc = s1g_decode_fdbaq(c, b.user_data, b.ud_offsets, b.ud_lengths, b.num_quads, L, b.max_user_data_bytes,
+                              b.max_num_quads, prec, c->ws, N, b.status);
         if (rc) return rc;
+        rc = do_eccm(c, b.eccm, prec, c->ws, N, b.out, b.ld_out, b.eccm_flags);
+    } else {
+        rc = s1g_decode_fdbaq(c, b.user_data, b.ud_offsets, b.ud_lengths, b.num_quads, L, b.max_user_data_bytes,
+                              b.max_num_quads, prec, b.out, b.ld_out, b.status);
     }
+    if (rc) return rc;
     rc = do_range(c, b.range, prec, b.out, b.ld_out, c->ws, N);
     if (rc) return rc;
     rc = do_azimuth_fft(c, prec, L, N, c->ws, N, 0);

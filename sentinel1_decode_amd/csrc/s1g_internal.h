@@ -61,8 +61,12 @@ int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, cons
                    int64_t ld_in, void* out, int64_t ld_out);
 
 // ECCM.
-int launch_eccm(hipStream_t stream, int prec, const s1g_eccm_desc& d, const DevFft* fdet, int det_split,
-                const DevFft& fwin, void* data, int64_t ld, uint8_t* flags, void* scratch);
+// Out of place (windows overlap, so workgroups would race in place).
+// fN: plan of length N (twiddles only), fsub: plan of N / r, fw: window plan;
+// partial: n_lines * r values of the detection maxima.
+int launch_eccm(hipStream_t stream, int prec, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,
+                const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
+                void* partial);
 
 // Autofocus.
 int launch_af(hipStream_t stream, int prec, int n_lines, int n_samples, const void* rd, int64_t ld,

@@ -163,3 +163,53 @@ def test_focus_burst_f32_vs_f64(ctx, small_burst):
     b = db.focus(ctx, F32).cpu().numpy()
     torch.cuda.synchronize()
     assert rel_l2(b, a) < 1e-5
+
+
+def _rfi_lines(seed=21, L=24, N=3000):
+    """Random clutter lines with strong tones on a third of them, and ECCM
+    thresholds placed far from every decision the data produces."""
+    rng = np.random.default_rng(seed)
+    x = (rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N))) * 10
+    n = np.arange(N)
+    for l in range(0, L, 3):
+        for _ in range(3):
+            x[l] += 300 * np.exp(2j * np.pi * (rng.uniform(-0.45, 0.45) * n + rng.uniform()))
+    spec_max = np.abs(np.fft.fft(x, axis=1)).max(1)
+    det = 0.5 * (spec_max[1::3].max() + spec_max[0::3].min())
+    return x, det, 2000.0
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_eccm_vs_oracle(ctx, prec):
+    x, det, mit = _rfi_lines()
+    ref, rflags = oracle.eccm(x, det, mit)
+    dt = torch.complex128 if prec == F64 else torch.complex64
+    d = torch.from_numpy(x).to(dt).cuda()
+    flags = torch.zeros(x.shape[0], dtype=torch.uint8, device="cuda")
+    desc = capi.EccmDesc(x.shape[0], x.shape[1], 128, 64, det, mit)
+    ctx.eccm(d, desc, flags)
+    torch.cuda.synchronize()
+    assert np.array_equal(flags.cpu().numpy(), rflags)
+    assert rflags.sum() == 8
+    tol = 1e-12 if prec == F64 else 1e-5
+    assert rel_l2(d.cpu().numpy(), ref) < tol
+
+
+def test_focus_burst_with_eccm_f64(ctx):
+    spec = synth.config0(n_lines=160, n_samples=4096, grid_spacing=(300, 40), seed=13)
+    raw = spec.raw()
+    mask = synth.add_rfi(raw, spec, seed=3, frac=0.2, run=(10, 20))
+    burst = synth.encode_burst(spec, raw)
+    dec, _ = oracle.decode_packets(burst.stream, burst.ud_offsets, burst.ud_lengths, burst.num_quads)
+    smax = np.abs(np.fft.fft(dec, axis=1)).max(1)
+    det = 0.5 * (smax[~mask].max() + smax[mask].min())
+    assert smax[~mask].max() < det < smax[mask].min()
+    mit = 4000.0
+    ref = oracle_chain(burst, eccm=(det, mit))
+    assert np.array_equal(ref["eccm_flags"].astype(bool), mask)
+    db = pipeline.from_synth(burst)
+    flags = torch.zeros(spec.n_lines, dtype=torch.uint8, device="cuda")
+    slc = db.focus(ctx, F64, eccm=capi.EccmDesc(spec.n_lines, spec.n_samples, 128, 64, det, mit), eccm_flags=flags)
+    torch.cuda.synchronize()
+    assert np.array_equal(flags.cpu().numpy().astype(bool), mask)
+    assert rel_l2(slc.cpu().numpy(), ref["slc"]) < 1e-10
