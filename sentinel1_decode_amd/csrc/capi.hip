@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <new>
@@ -40,8 +41,10 @@ struct s1g_ctx {
     size_t cols_n = 0;
     void* eccm_scratch = nullptr;
     size_t eccm_scratch_bytes = 0;
-    double* af_scalars = nullptr;  // autofocus partial sums
+    double* af_scalars = nullptr;  // autofocus scalars and partial sums
     size_t af_n = 0;
+    void* af_img = nullptr;
+    size_t af_img_bytes = 0;
     std::map<std::string, hipGraphExec_t> graphs;
     std::vector<hipGraph_t> graph_objs;
     // stage timers
@@ -365,6 +368,7 @@ int s1g_destroy(s1g_ctx* c) {
     hipFree(c->cols);
     hipFree(c->eccm_scratch);
     hipFree(c->af_scalars);
+    hipFree(c->af_img);
     if (c->own) hipStreamDestroy(c->own);
     delete c;
     return S1G_OK;
@@ -608,27 +612,137 @@ int s1g_focus_burst(s1g_ctx* c, const s1g_burst_desc* b, int prec) {
     return burst_chain(c, *b, prec);
 }
 
+// Everything a burst chain allocates or builds on the host (FFT plans, range
+// filters, workspace, column table, ECCM scratch), done ahead of a capture.
+static int prepare_burst(s1g_ctx* c, const s1g_burst_desc& b, int prec, size_t* ws_need, size_t* cols_need) {
+    const int L = b.range.n_lines, N = b.range.n_samples;
+    if (L <= 1 || N <= 0) return fail(c, S1G_EARG, "burst needs >= 2 lines%s");
+    const int nc = (int)floor(b.range.fs_mhz * b.range.txpl_us);
+    if (nc < 2) return fail(c, S1G_EARG, "replica chirp shorter than 2 samples%s");
+    const int m = range_block_len(prec, N, nc);
+    if (m < 0) return fail(c, S1G_EUNSUPPORTED, "replica chirp longer than the overlap-save block%s");
+    const DevFft *fm, *fl;
+    int rc = get_plan(c, m, &fm);
+    if (!rc) rc = get_plan(c, L, &fl);
+    void* h;
+    if (!rc) rc = get_filter(c, b.range, prec, nc, fm, &h);
+    if (!rc && b.use_eccm) {
+        const DevFft* f;
+        rc = get_fft(c, N, &f);
+        if (!rc) rc = get_plan(c, b.eccm.fft_size, &f);
+        for (int k : {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32}) {
+            if (rc || N % k) continue;
+            FftPlan p;
+            if ((size_t)(N / k) * esize(prec) <= 128 * 1024 && factorise(N / k, p)) {
+                rc = get_plan(c, N / k, &f);
+                if (!rc) rc = ensure(c, &c->eccm_scratch, &c->eccm_scratch_bytes, sizeof(double) * (size_t)L * k);
+                break;
+            }
+        }
+    }
+    *ws_need = std::max(*ws_need, esize(prec) * (size_t)L * N);
+    *cols_need = std::max(*cols_need, sizeof(AzColumn) * (size_t)N);
+    return rc;
+}
+
 int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, int flags) {
     if (!c || (!descs && n)) return S1G_EARG;
     if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (n == 0) return S1G_OK;
+    size_t ws_need = 0, cols_need = 0;
     for (int i = 0; i < n; ++i) {
-        int rc = burst_chain(c, descs[i], prec);
+        int rc = prepare_burst(c, descs[i], prec, &ws_need, &cols_need);
         if (rc) return rc;
     }
-    (void)flags;
+    int rc = ensure(c, &c->ws, &c->ws_bytes, ws_need);
+    if (!rc) rc = ensure(c, (void**)&c->cols, &c->cols_n, cols_need);
+    if (rc) return rc;
+    if (!(flags & S1G_BATCH_GRAPH)) {
+        for (int i = 0; i < n; ++i) {
+            rc = burst_chain(c, descs[i], prec);
+            if (rc) return rc;
+        }
+        return S1G_OK;
+    }
+    // one hipGraph per distinct descriptor set
+    std::string key((const char*)descs, sizeof(s1g_burst_desc) * (size_t)n);
+    key.append((const char*)&prec, sizeof prec);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        hipStream_t user = c->stream;
+        HIPCHK(c, hipStreamSynchronize(user));  // filters built above are complete
+        c->stream = c->own;
+        c->capturing = true;
+        hipError_t e = hipStreamBeginCapture(c->own, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            c->capturing = false;
+            c->stream = user;
+            return hip_fail(c, e, "hipStreamBeginCapture");
+        }
+        for (int i = 0; i < n && !rc; ++i) rc = burst_chain(c, descs[i], prec);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(c->own, &g);
+        c->capturing = false;
+        c->stream = user;
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(g);
+            return hip_fail(c, e, "hipGraphInstantiate");
+        }
+        c->graph_objs.push_back(g);
+        it = c->graphs.emplace(key, ge).first;
+    }
+    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    return S1G_OK;
+}
+
+static int af_run(s1g_ctx* c, int prec, int L, int N, const void* rd, int64_t ld, const double* phase, void* img,
+                  double* entropy, double* grad) {
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (L < 2 || N < 1 || !rd || !phase || ld < N || !entropy) return fail(c, S1G_EARG, "bad autofocus arguments%s");
+    const DevFft* fl;
+    int rc = get_plan(c, L, &fl);
+    if (rc) return rc;
+    const size_t img_bytes = esize(prec) * (size_t)L * N;
+    const size_t work = sizeof(double) * (af_work_doubles(L, N, prec) + 4);
+    rc = ensure(c, &c->af_img, &c->af_img_bytes, img_bytes);
+    if (!rc) rc = ensure(c, (void**)&c->af_scalars, &c->af_n, work);
+    if (rc) return rc;
+    double* scal = c->af_scalars;
+    double* wk = c->af_scalars + 4;
+    {
+        StageTimer tm(c, S1G_STAGE_AUTOFOCUS);
+        rc = launch_af(c->stream, prec, L, N, rd, ld, phase, *fl, c->af_img, wk, grad != nullptr, grad, scal);
+    }
+    if (rc) return check_launch(c, rc, "autofocus");
+    if (img) {
+        const size_t row = esize(prec) * (size_t)N;
+        HIPCHK(c, hipMemcpy2DAsync(img, row, c->af_img, row, row, L, hipMemcpyDeviceToDevice, c->stream));
+    }
+    double h[3];
+    HIPCHK(c, hipMemcpyAsync(h, scal, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *entropy = h[2];
     return S1G_OK;
 }
 
 int s1g_af_forward(s1g_ctx* c, int prec, int L, int N, const void* rd, int64_t ld, const double* phase, void* img,
                    double* entropy) {
     if (!c) return S1G_EARG;
-    return fail(c, S1G_EUNSUPPORTED, "autofocus is not available in this build%s");
+    return af_run(c, prec, L, N, rd, ld, phase, img, entropy, nullptr);
 }
 
 int s1g_af_adjoint(s1g_ctx* c, int prec, int L, int N, const void* rd, int64_t ld, const double* phase,
                    double* entropy, double* grad) {
     if (!c) return S1G_EARG;
-    return fail(c, S1G_EUNSUPPORTED, "autofocus is not available in this build%s");
+    if (!grad) return fail(c, S1G_EARG, "null gradient%s");
+    return af_run(c, prec, L, N, rd, ld, phase, nullptr, entropy, grad);
 }
 
 }  // extern "C"

@@ -68,9 +68,11 @@ int launch_eccm(hipStream_t stream, int prec, const s1g_eccm_desc& d, const DevF
                 const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
                 void* partial);
 
-// Autofocus.
+// Autofocus.  work: af_work_doubles() doubles; scal: 3 doubles (S, T, E);
+// img: n_lines * n_samples complex (dense).
+size_t af_work_doubles(int n_lines, int n_samples, int prec);
 int launch_af(hipStream_t stream, int prec, int n_lines, int n_samples, const void* rd, int64_t ld,
-              const double* phase, const DevFft& fl, void* img, double* partial, int want_grad, double* grad,
-              double* scalars);
+              const double* phase, const DevFft& fl, void* img, double* work, int want_grad, double* grad,
+              double* scal);
 
 }  // namespace s1g
