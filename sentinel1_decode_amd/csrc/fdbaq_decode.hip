@@ -10,9 +10,11 @@
 //   stage   the packet's user data is copied into LDS as big-endian 32-bit
 //           words (all 64 lanes, coalesced);
 //   phase 1 the block structure is walked channel by channel (IE, IO, QE, QO):
-//           BRC/THIDX headers are read, and for each block the 64 lanes
-//           evaluate the symbol length at 64 consecutive bit positions while
-//           a uniform readlane chain hops from symbol to symbol; this yields
+//           BRC/THIDX headers are read and the end of each block's symbols
+//           is found by a speculative, self-synchronising walk (walk_block):
+//           each lane decodes the symbols starting in its own slice of the
+//           block, neighbours then agree on true slice entries, and a prefix
+//           sum of symbol counts locates the block's last symbol; this yields
 //           the start bit of every (channel, block) and all error checks;
 //   phase 2 lanes take (channel, block) pairs and decode their symbols
 //           serially from the recorded starts, reconstruct the samples in
@@ -59,6 +61,10 @@ __constant__ double c_sf[256] = {
     230.93, 232.18, 233.43, 234.69, 235.94, 237.19, 238.45, 239.70, 240.95, 242.21, 243.46, 244.71, 245.97,
     247.22, 248.47, 249.73, 250.98, 252.23, 253.49, 254.74, 255.99, 255.99};
 
+// Zero words staged after a packet: a walk region reaches at most 64 * 32
+// bits beyond the last checked position, plus the 3-word window.
+constexpr int kPadWords = 72;
+
 // `n` (1..32) bits starting at bit `pos` of the big-endian word array.
 __device__ __forceinline__ uint32_t peek_bits(const uint32_t* w, int64_t pos, int n) {
     const int64_t wi = pos >> 5;
@@ -78,6 +84,118 @@ __device__ __forceinline__ double recon(int brc, int thidx, int sign, int m) {
     return sign ? -v : v;
 }
 
+// Bits [o, o + 11) of the 96-bit big-endian window (a, b, c), o + 11 <= 96.
+__device__ __forceinline__ uint32_t win11(uint32_t a, uint32_t b, uint32_t c, int o) {
+    const uint64_t hi = ((uint64_t)a << 32) | b;
+    const uint64_t lo = ((uint64_t)b << 32) | c;
+    const uint64_t v = o < 32 ? hi : lo;
+    const int sh = o & 31;
+    return (uint32_t)((v << sh) >> 53);
+}
+
+// Value of the lane to the left (lane 0 gets 0): DPP wave_shr:1.
+__device__ __forceinline__ int from_left(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
+
+// Inclusive prefix sum across the 64 lanes: row_shr 1, 2, 4, 8 inside each
+// row of 16 lanes, then the totals of the lower rows (read from lanes 15,
+// 31, 47) are added.
+__device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
+    const int r0 = __builtin_amdgcn_readlane(x, 15);
+    const int r1 = __builtin_amdgcn_readlane(x, 31);
+    const int r2 = __builtin_amdgcn_readlane(x, 47);
+    const int row = lane >> 4;
+    return x + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+}
+
+__device__ __forceinline__ int sym_len(uint32_t a, uint32_t b, uint32_t c, int o, int brc) {
+    return 1 + fdbaq_code_len(brc, win11(a, b, c, o) & 0x3FFu);
+}
+
+// Position after `nsym` symbols of bit-rate code `brc` starting at bit
+// `start` (uniform across the wave).  Lane l owns the slice
+// [start + l*delta, start + (l+1)*delta) of a region of 64*delta bits:
+//   1. it decodes speculatively from its slice start, recording in `mask`
+//      the slice offsets where its symbols start, until it leaves the slice;
+//   2. its true entry is its left neighbour's exit; if that offset is one it
+//      already visited, its walk has synchronised and its exit stands,
+//      otherwise it walks again from the entry (a slice that a symbol from
+//      the left covers entirely passes the entry on as its exit); repeat
+//      until no exit changes;
+//   3. a prefix sum of per-lane symbol counts finds the lane holding the
+//      nsym-th symbol, which walks to its end.
+// Regions repeat when one does not hold all nsym symbols.  delta <= 32.
+// Reads stay below word `wlim` (the staged words); once `start` passes
+// `nbits` the block cannot end inside the packet and the walk stops there.
+__device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, int64_t nbits, int64_t start, int brc,
+                              int nsym, int delta, int lane) {
+    for (;;) {
+        if (start > nbits) return start;
+        const int64_t s = start + (int64_t)lane * delta;
+        const int64_t w0 = s >> 5;
+        const uint32_t a = w0 < wlim ? words[w0] : 0u;
+        const uint32_t b = w0 + 1 < wlim ? words[w0 + 1] : 0u;
+        const uint32_t c = w0 + 2 < wlim ? words[w0 + 2] : 0u;
+        const int base = (int)(s & 31);
+        uint32_t mask = 0;
+        int o = 0;
+        while (o < delta) {
+            mask |= 1u << o;
+            o += sym_len(a, b, c, base + o, brc);
+        }
+        int wex = o;  // exit of the recorded walk
+        int ex = o;   // exit handed to the right neighbour
+        int entry = 0;
+        // Exits settle left to right, at least one more lane per round: 64
+        // rounds suffice, the bound only guards against a malformed table.
+        for (int round = 0; round <= 64; ++round) {
+            const int up = from_left(ex);
+            entry = lane == 0 ? 0 : up - delta;
+            bool changed = false;
+            int nex;
+            if (entry >= delta) {
+                nex = entry;  // a symbol from the left covers this whole slice
+            } else {
+                if (!((mask >> entry) & 1u)) {
+                    mask = 0;
+                    o = entry;
+                    while (o < delta) {
+                        mask |= 1u << o;
+                        o += sym_len(a, b, c, base + o, brc);
+                    }
+                    wex = o;
+                    changed = true;
+                }
+                nex = wex;
+            }
+            changed |= nex != ex;
+            ex = nex;
+            if (__ballot(changed) == 0) break;
+        }
+        const int cnt = entry >= delta ? 0 : __popc(mask >> entry);
+        const int P = wave_inclusive_sum(cnt, lane);
+        const int total = __builtin_amdgcn_readlane(P, 63);
+        if (total >= nsym) {
+            const unsigned long long bal = __ballot(P >= nsym);
+            const int ls = __ffsll((long long)bal) - 1;
+            int endrel = 0;
+            if (lane == ls) {
+                const int k = nsym - (P - cnt);
+                o = entry;
+                for (int i = 0; i < k; ++i) o += sym_len(a, b, c, base + o, brc);
+                endrel = o;
+            }
+            endrel = __builtin_amdgcn_readlane(endrel, ls);
+            return start + (int64_t)ls * delta + endrel;
+        }
+        nsym -= total;
+        start += (int64_t)63 * delta + __builtin_amdgcn_readlane(ex, 63);
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
                                                           const uint64_t* __restrict__ offs,
@@ -90,8 +208,8 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
     const int lane = threadIdx.x;
     if (p >= n_packets) return;
 
-    uint32_t* words = smem;                                      // max_words + 8
-    int32_t* bstart = (int32_t*)(smem + max_words + 8);          // 4 * max_blocks
+    uint32_t* words = smem;                                      // max_words + kPadWords
+    int32_t* bstart = (int32_t*)(smem + max_words + kPadWords);  // 4 * max_blocks
     uint8_t* bbrc = (uint8_t*)(bstart + 4 * max_blocks);         // max_blocks
     uint8_t* bthr = bbrc + max_blocks;                           // max_blocks
 
@@ -110,7 +228,7 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
         const int off = (int)(au & 3);
         const int64_t last_al = ((int64_t)off + nbytes - 1) >> 2;  // last aligned word touching the packet
         const int nwords = (int)((nbytes + 3) >> 2);
-        for (int i = lane; i < nwords + 8; i += 64) {
+        for (int i = lane; i < nwords + kPadWords; i += 64) {
             uint32_t be = 0;
             if (i < nwords) {
                 uint32_t lo = (i <= last_al) ? al[i] : 0u;
@@ -128,7 +246,9 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
 
     // ---- phase 1: block structure (uniform control flow)
     const int64_t nbits = 8 * nbytes;
+    const int64_t nwords_staged = (nbytes + 3) / 4 + kPadWords;
     int64_t pos = 0;
+    int prev_len = 0;
     for (int ch = 0; ch < 4 && st == S1G_PKT_OK; ++ch) {
         for (int b = 0; b < nb && st == S1G_PKT_OK; ++b) {
             int brc;
@@ -158,19 +278,13 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
                 pos += 8;
             }
             if (lane == 0) bstart[ch * max_blocks + b] = (int32_t)pos;
-            int left = (b == nb - 1) ? nq - 128 * (nb - 1) : 128;
-            while (left > 0) {
-                // symbol length (sign + code) at bit pos + lane
-                const uint32_t bits = peek_bits(words, pos + lane, 11);
-                const int len = 1 + fdbaq_code_len(brc, bits & 0x3FFu);
-                int x = 0;
-                while (left > 0 && x < 64) {
-                    x += __builtin_amdgcn_readlane(len, x);
-                    --left;
-                }
-                pos += x;
-                if (pos > nbits) break;  // past the end: the packet is truncated
-            }
+            const int nsym = (b == nb - 1) ? nq - 128 * (nb - 1) : 128;
+            // slice width from the previous block of this channel (or a BRC guess)
+            const int est = b ? prev_len : nsym * (brc + 3);
+            const int delta = min(max((est + 63) / 64 + 1, 2), 32);
+            const int64_t end = walk_block(words, nwords_staged, nbits, pos, brc, nsym, delta, lane);
+            prev_len = (int)(end - pos);
+            pos = end;
             if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
         }
         pos = (pos + 15) & ~(int64_t)15;
@@ -213,7 +327,7 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     if (n_packets <= 0) return 0;
     const int max_words = (max_bytes + 3) / 4;
     const int max_blocks = (max_quads + 127) / 128 + 1;
-    size_t lds = sizeof(uint32_t) * (size_t)(max_words + 8) + sizeof(int32_t) * 4 * (size_t)max_blocks +
+    size_t lds = sizeof(uint32_t) * (size_t)(max_words + kPadWords) + sizeof(int32_t) * 4 * (size_t)max_blocks +
                  2 * (size_t)max_blocks;
     lds = (lds + 15) & ~(size_t)15;
     if (lds > 160 * 1024) return -1;
