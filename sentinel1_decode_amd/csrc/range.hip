@@ -38,13 +38,36 @@ __global__ void chirp_filter_seed_kernel(cpx<double>* __restrict__ g, int m, int
     g[u] = mk(scale * cs, scale * sn);
 }
 
+// Register-pass kernels: block length M = R0 R1 R2 handled by NT threads.
+struct OsShape {
+    int m, r0, r1, r2, nt;
+};
+
+// The register-pass configuration for (precision, M), or m = 0 when M is
+// handled by the generic LDS kernel.
+static OsShape os_shape(int prec, int m) {
+    if (prec == S1G_F32 && m == 16384) return {16384, 32, 32, 16, 512};
+    if (prec == S1G_F32 && m == 8192) return {8192, 32, 16, 16, 256};
+    if (prec == S1G_F64 && m == 8192) return {8192, 32, 16, 16, 256};
+    return {0, 0, 0, 0, 0};
+}
+
+// h[dst(f)] = conj(G[f]) / m.  For the register-pass kernels the filter is
+// stored in the order their last forward pass leaves the spectrum: bin
+// f = k0 + R0 k1 + R0 R1 k2 sits at k2 (m / R2) + k0 R1 + k1.
 template <typename T>
-__global__ void filter_finalize_kernel(const cpx<double>* __restrict__ G, cpx<T>* __restrict__ h, int m) {
+__global__ void filter_finalize_kernel(const cpx<double>* __restrict__ G, cpx<T>* __restrict__ h, int m, int r0,
+                                       int r1, int r2) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m) return;
     const double inv = 1.0 / (double)m;
     const cpx<double> v = G[f];
-    h[f] = mk((T)(v.x * inv), (T)(-v.y * inv));
+    int dst = f;
+    if (r0) {
+        const int k0 = f % r0, k1 = (f / r0) % r1, k2 = f / (r0 * r1);
+        dst = k2 * (m / r2) + k0 * r1 + k1;
+    }
+    h[dst] = mk((T)(v.x * inv), (T)(-v.y * inv));
 }
 
 // scratch: 3 * M complex doubles.
@@ -59,12 +82,13 @@ int launch_range_filter(hipStream_t s, const DevFft& fm, double txpsf, double tx
     cpx<double>* g3 = g2 + m;
     hipLaunchKernelGGL(chirp_filter_seed_kernel, gr, b, 0, s, g1, m, nc, txpsf, txprr * 0.5, txpl, scale);
     launch_fft_global(s, S1G_F64, fm, g1, g2, g3, 1, m, 0);  // G = FFT_M(g) in g2
+    const OsShape sh = os_shape(prec, m);
     if (prec == S1G_F32)
         hipLaunchKernelGGL(filter_finalize_kernel<float>, gr, b, 0, s, (const cpx<double>*)g2, (cpx<float>*)h_out,
-                           m);
+                           m, sh.r0, sh.r1, sh.r2);
     else
         hipLaunchKernelGGL(filter_finalize_kernel<double>, gr, b, 0, s, (const cpx<double>*)g2,
-                           (cpx<double>*)h_out, m);
+                           (cpx<double>*)h_out, m, sh.r0, sh.r1, sh.r2);
     return (int)hipGetLastError();
 }
 
@@ -102,6 +126,156 @@ __global__ __launch_bounds__(NT) void range_os_kernel(const cpx<T>* __restrict__
     for (int k = tid; k < cnt; k += NT) orow[j0 + k] = buf[perm[k]];
 }
 
+
+// LDS index with one pad element per 16, so that the last pass's
+// contiguous 16/32-element reads by neighbouring lanes hit distinct banks.
+__device__ __forceinline__ int os_pad(int a) { return a + (a >> 4); }
+
+// Overlap-save block with three register-resident radix passes.
+// Forward (in-place DIF on the index n = n' + S0 n0, S0 = M / R0, S1 = S0 / R1):
+//   pass 0: DFT_R0 over n0 for each n' (loaded straight from global memory),
+//           times w_M^{n' k0};
+//   pass 1: per k0, DFT_R1 over the stride-S1 elements, times w_S0^{r k1};
+//   pass 2: per (k0, k1), DFT_R2 over R2 contiguous elements; bin
+//           f = k0 + R0 k1 + R0 R1 k2 is in registers, multiplied by the
+//           filter (stored in that order) and transformed back by the
+//           conjugate pass at once.
+// The inverse passes mirror the forward ones in reverse order, so the last
+// one holds outputs n' + S0 n0 and stores them straight to global memory.
+// Twiddles: w_M^e = lo[e mod 128] * hi[e / 128] and w_S0^{r k1} from a
+// [k1][r] table, all in LDS.
+template <typename T, int M, int R0, int R1, int R2, int NT>
+__global__ __launch_bounds__(NT) void range_os3_kernel(const cpx<T>* __restrict__ in, int64_t ld_in,
+                                                       cpx<T>* __restrict__ out, int64_t ld_out, int n_lines, int N,
+                                                       int cshift, int V, int nblk, const cpx<T>* __restrict__ tw,
+                                                       const cpx<T>* __restrict__ hs) {
+    constexpr int S0 = M / R0, S1 = S0 / R1;
+    static_assert(S1 == R2 && R0 * R1 * R2 == M, "three passes cover M");
+    constexpr int D0 = S0, D1 = M / R1, D2 = M / R2;
+    static_assert(D0 == NT && D1 % NT == 0 && D2 % NT == 0, "whole butterflies per thread");
+    constexpr int P1 = D1 / NT, P2 = D2 / NT;
+    constexpr int E = M / NT;
+    static_assert(E == R0 && E == P1 * R1 && E == P2 * R2, "E elements per thread in every pass");
+    constexpr int NHI = M / 128;
+    constexpr int LDATA = M + M / 16;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = (cpx<T>*)smem_raw;
+    cpx<T>* tlo = buf + LDATA;
+    cpx<T>* thi = tlo + 128;
+    cpx<T>* t1 = thi + NHI;  // [k1][r], R1 * S1 entries
+    const int tid = threadIdx.x;
+    const int line = blockIdx.x / nblk;
+    const int blk = blockIdx.x - line * nblk;
+    if (line >= n_lines) return;
+    const int j0 = blk * V;
+    const cpx<T>* row = in + (int64_t)line * ld_in;
+
+    cpx<T> a[E];
+    // pass 0 input: n = tid + S0 n0, in [j0 - cshift, j0 - cshift + M)
+    {
+        const int src0 = j0 - cshift + tid;
+#pragma unroll
+        for (int n0 = 0; n0 < R0; ++n0) {
+            const int src = src0 + S0 * n0;
+            a[n0] = (src >= 0 && src < N) ? row[src] : mk((T)0, (T)0);
+        }
+    }
+    for (int i = tid; i < 128; i += NT) tlo[i] = tw[i];
+    for (int i = tid; i < NHI; i += NT) thi[i] = tw[128 * i];
+    for (int i = tid; i < R1 * S1; i += NT) {
+        const int k1 = i / S1, r = i - k1 * S1;
+        t1[i] = tw[((r * k1) % S0) * R0];
+    }
+    __syncthreads();
+
+    // ---- forward pass 0
+    Dft<T, R0, false>::run(a);
+#pragma unroll
+    for (int k0 = 1; k0 < R0; ++k0) {
+        const int e = (tid * k0) & (M - 1);
+        a[k0] = cmul(a[k0], cmul(tlo[e & 127], thi[e >> 7]));
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < R0; ++k0) buf[os_pad(tid + S0 * k0)] = a[k0];
+    __syncthreads();
+
+    // ---- forward pass 1
+#pragma unroll
+    for (int p = 0; p < P1; ++p) {
+        const int d = tid + NT * p;
+        const int k0 = d / S1, r = d - k0 * S1;
+        const int base = k0 * S0 + r;
+        cpx<T>* v = a + p * R1;
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = buf[os_pad(base + S1 * n1)];
+        Dft<T, R1, false>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], t1[k1 * S1 + r]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) buf[os_pad(base + S1 * k1)] = v[k1];
+    }
+    __syncthreads();
+
+    // ---- pass 2, filter, inverse pass 2
+#pragma unroll
+    for (int p = 0; p < P2; ++p) {
+        const int d = tid + NT * p;
+        const int base = d * R2;
+        cpx<T>* v = a + p * R2;
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) v[n2] = buf[os_pad(base + n2)];
+        Dft<T, R2, false>::run(v);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[k2] = cmul(v[k2], hs[k2 * D2 + d]);
+        Dft<T, R2, true>::run(v);
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) buf[os_pad(base + n2)] = v[n2];
+    }
+    __syncthreads();
+
+    // ---- inverse pass 1
+#pragma unroll
+    for (int p = 0; p < P1; ++p) {
+        const int d = tid + NT * p;
+        const int k0 = d / S1, r = d - k0 * S1;
+        const int base = k0 * S0 + r;
+        cpx<T>* v = a + p * R1;
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) v[k1] = buf[os_pad(base + S1 * k1)];
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmulc(v[k1], t1[k1 * S1 + r]);
+        Dft<T, R1, true>::run(v);
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) buf[os_pad(base + S1 * n1)] = v[n1];
+    }
+    __syncthreads();
+
+    // ---- inverse pass 0 and store
+#pragma unroll
+    for (int k0 = 0; k0 < R0; ++k0) a[k0] = buf[os_pad(tid + S0 * k0)];
+#pragma unroll
+    for (int k0 = 1; k0 < R0; ++k0) {
+        const int e = (tid * k0) & (M - 1);
+        a[k0] = cmulc(a[k0], cmul(tlo[e & 127], thi[e >> 7]));
+    }
+    Dft<T, R0, true>::run(a);
+    cpx<T>* orow = out + (int64_t)line * ld_out;
+    const int cnt = min(V, N - j0);
+#pragma unroll
+    for (int n0 = 0; n0 < R0; ++n0) {
+        const int k = tid + S0 * n0;
+        if (k < cnt) orow[j0 + k] = a[n0];
+    }
+}
+
+template <typename T, int M, int R0, int R1, int R2, int NT>
+static void launch_os3(hipStream_t s, dim3 grid, const void* in, int64_t ld_in, void* out, int64_t ld_out,
+                       int n_lines, int N, int cshift, int V, int nblk, const DevFft& fm, const void* h) {
+    const size_t lds = sizeof(cpx<T>) * (size_t)(M + M / 16 + 128 + M / 128 + R1 * (M / R0 / R1));
+    hipLaunchKernelGGL((range_os3_kernel<T, M, R0, R1, R2, NT>), grid, dim3(NT), lds, s, (const cpx<T>*)in, ld_in,
+                       (cpx<T>*)out, ld_out, n_lines, N, cshift, V, nblk, tw_of<T>(fm), (const cpx<T>*)h);
+}
+
 int launch_range_os(hipStream_t s, int prec, const DevFft& fm, const void* in, int64_t ld_in, void* out,
                     int64_t ld_out, int n_lines, int n_samples, int nc, const void* h) {
     const int M = fm.plan.n;
@@ -110,6 +284,19 @@ int launch_range_os(hipStream_t s, int prec, const DevFft& fm, const void* in, i
     const int nblk = (n_samples + V - 1) / V;
     const int cshift = (nc + 1) / 2;
     dim3 grid((unsigned)(nblk * n_lines));
+    const OsShape sh = os_shape(prec, M);
+    if (sh.m) {
+        if (prec == S1G_F32 && M == 16384)
+            launch_os3<float, 16384, 32, 32, 16, 512>(s, grid, in, ld_in, out, ld_out, n_lines, n_samples, cshift, V,
+                                                       nblk, fm, h);
+        else if (prec == S1G_F32)
+            launch_os3<float, 8192, 32, 16, 16, 256>(s, grid, in, ld_in, out, ld_out, n_lines, n_samples, cshift, V,
+                                                      nblk, fm, h);
+        else
+            launch_os3<double, 8192, 32, 16, 16, 256>(s, grid, in, ld_in, out, ld_out, n_lines, n_samples, cshift, V,
+                                                       nblk, fm, h);
+        return (int)hipGetLastError();
+    }
     constexpr int NT = 512;
     if (prec == S1G_F32) {
         size_t lds = sizeof(cpx<float>) * (size_t)M;

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 namespace s1g {
 
 // ---------------------------------------------------------------- complex
@@ -184,6 +186,64 @@ struct Dft {
         odd_out<T, R, INV, 1>(a, s, d, x0);
     }
 };
+
+// Multiply by the compile-time twiddle w_R^E (forward: e^{-2 pi i E/R};
+// INV: e^{+2 pi i E/R}); trivial angles cost no multiplies.
+template <typename T, int R, int E, bool INV>
+__device__ __forceinline__ cpx<T> cwk(cpx<T> x) {
+    constexpr int e = ((E % R) + R) % R;
+    if constexpr (e == 0) {
+        return x;
+    } else if constexpr (2 * e == R) {
+        return mk(-x.x, -x.y);
+    } else if constexpr (4 * e == R) {
+        return INV ? mk(-x.y, x.x) : mk(x.y, -x.x);
+    } else if constexpr (4 * e == 3 * R) {
+        return INV ? mk(x.y, -x.x) : mk(-x.y, x.x);
+    } else {
+        constexpr T c = (T)turn_cos(e, R);
+        constexpr T sn = (T)(INV ? turn_sin(e, R) : -turn_sin(e, R));
+        return mk(x.x * c - x.y * sn, x.x * sn + x.y * c);
+    }
+}
+
+// Cooley-Tukey DFT of length R = A * B in registers: n = b + B a,
+// k = k1 + A k2;  X[k1 + A k2] = sum_b w_B^{b k2} w_R^{b k1} DFT_A(x[b + B .])[k1].
+// Index packs are compile-time, so every twiddle is a constant.
+template <typename T, int A, int B, bool INV>
+struct DftCT {
+    static constexpr int R = A * B;
+    template <int b, int... as>
+    static __device__ __forceinline__ void inner(const cpx<T>* x, cpx<T>* y, std::integer_sequence<int, as...>) {
+        cpx<T> t[A] = {x[b + B * as]...};
+        Dft<T, A, INV>::run(t);
+        ((y[b * A + as] = cwk<T, R, b * as, INV>(t[as])), ...);
+    }
+    template <int... bs>
+    static __device__ __forceinline__ void inners(const cpx<T>* x, cpx<T>* y, std::integer_sequence<int, bs...>) {
+        (inner<bs>(x, y, std::make_integer_sequence<int, A>{}), ...);
+    }
+    template <int k1, int... bs>
+    static __device__ __forceinline__ void outer(const cpx<T>* y, cpx<T>* x, std::integer_sequence<int, bs...>) {
+        cpx<T> t[B] = {y[bs * A + k1]...};
+        Dft<T, B, INV>::run(t);
+        ((x[k1 + A * bs] = t[bs]), ...);
+    }
+    template <int... k1s>
+    static __device__ __forceinline__ void outers(const cpx<T>* y, cpx<T>* x, std::integer_sequence<int, k1s...>) {
+        (outer<k1s>(y, x, std::make_integer_sequence<int, B>{}), ...);
+    }
+    static __device__ __forceinline__ void run(cpx<T>* x) {
+        cpx<T> y[R];
+        inners(x, y, std::make_integer_sequence<int, B>{});
+        outers(y, x, std::make_integer_sequence<int, A>{});
+    }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 16, INV> : DftCT<T, 4, 4, INV> {};
+template <typename T, bool INV>
+struct Dft<T, 32, INV> : DftCT<T, 8, 4, INV> {};
 
 // Radices the LDS passes support; the host planner factors lengths over them.
 #define S1G_RADIX_LIST(X) X(2) X(3) X(4) X(5) X(7) X(8) X(11) X(13) X(17) X(19) X(23) X(29) X(31) X(37) X(41) X(43) X(47) X(53) X(59) X(61) X(67) X(71)

@@ -123,6 +123,23 @@ def test_range_compress(ctx, small_burst, small_chain, prec, tol):
     assert rel_l2(y.cpu().numpy(), small_chain["rc"]) < tol
 
 
+@pytest.mark.parametrize("prec,tol", [(F64, 1e-11), (F32, 2e-6)])
+@pytest.mark.parametrize("n_samples,fs", [(24000, 64.34523813), (21600, 50.04629632), (20400, 54.59595962)])
+def test_range_compress_iw_lengths(ctx, prec, tol, n_samples, fs):
+    """Full IW1/IW2/IW3 line lengths (two overlap-save blocks per line in
+    FP32, five in FP64), a few lines, against the oracle's literal
+    pad / FFT / multiply / IFFT / fftshift / crop."""
+    rng = np.random.default_rng(n_samples)
+    x = (rng.standard_normal((3, n_samples)) + 1j * rng.standard_normal((3, n_samples))) * 20
+    txpsf, txprr, txpl = -8.7e-1, 1.0789, 52.40481
+    ref = oracle.range_compress(x, txpsf, txprr, txpl, fs)
+    dt = torch.complex128 if prec == F64 else torch.complex64
+    d = torch.from_numpy(x).to(dt).cuda()
+    y = ctx.range_compress(d, capi.RangeDesc(3, n_samples, txpsf, txprr, txpl, fs))
+    torch.cuda.synchronize()
+    assert rel_l2(y.cpu().numpy(), ref) < tol
+
+
 @pytest.mark.parametrize("L", [192, 1480, 1420, 1520, 1024])
 def test_azimuth_fft_lengths(ctx, L):
     rng = np.random.default_rng(L)
