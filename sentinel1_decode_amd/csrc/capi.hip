@@ -420,8 +420,11 @@ int s1g_profile_read(s1g_ctx* c, double* ms, int64_t* launches, int reset) {
 
 int s1g_fdbaq_symbol(int brc, uint32_t code10, int* code_len, int* m_code) {
     if (brc < 0 || brc > 4 || code10 > 1023u) return S1G_EARG;
-    if (code_len) *code_len = fdbaq_code_len(brc, code10);
-    if (m_code) *m_code = fdbaq_mcode(brc, code10);
+    // the decoder's own table-driven rule
+    const int clo = fdbaq_clo(code10);
+    const uint32_t x4 = brc == 4 ? 1u : 0u;
+    if (code_len) *code_len = fdbaq_sym_len(code10, clo, fdbaq_len_table(brc), x4) - 1;
+    if (m_code) *m_code = fdbaq_mcode_tab(code10, clo, kMBase[brc], kMShift[brc], kMMask[brc], x4);
     return S1G_OK;
 }
 

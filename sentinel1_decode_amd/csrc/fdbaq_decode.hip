@@ -62,8 +62,18 @@ __constant__ double c_sf[256] = {
     247.22, 248.47, 249.73, 250.98, 252.23, 253.49, 254.74, 255.99, 255.99};
 
 // Zero words staged after a packet: a walk region reaches at most 64 * 32
-// bits beyond the last checked position, plus the 3-word window.
+// bits beyond the last checked position, plus the 4-word window.
 constexpr int kPadWords = 72;
+
+// Bits a lane's speculative walk starts before its slice, so that it has
+// usually fallen into step with the true symbol boundaries on arrival.
+// kOverlap + 32 (the widest slice) + 11 (the longest symbol) <= 64 keeps
+// every walk inside one 64-bit window.
+constexpr int kOverlap = 16;
+
+// Packets with more BAQ blocks than this are rejected (S1G_PKT_ERR_ARG):
+// the block BRCs are kept in 4 registers per lane.
+constexpr int kMaxBlocks = 256;
 
 // `n` (1..32) bits starting at bit `pos` of the big-endian word array.
 __device__ __forceinline__ uint32_t peek_bits(const uint32_t* w, int64_t pos, int n) {
@@ -73,24 +83,38 @@ __device__ __forceinline__ uint32_t peek_bits(const uint32_t* w, int64_t pos, in
     return (uint32_t)((v << sh) >> (64 - n));
 }
 
-// FP64 reconstruction (packet.cpp:678-693); the sign is applied by negation
-// so a zero magnitude with sign 1 gives -0.0 like pow(-1, 1) * 0.
-__device__ __forceinline__ double recon(int brc, int thidx, int sign, int m) {
-    double v;
-    if (thidx <= kThidxLimit[brc])
-        v = m < kMFlag[brc] ? (double)m : c_simple[brc][thidx];
-    else
-        v = __dmul_rn(c_nrl[brc][m], c_sf[thidx]);
-    return sign ? -v : v;
+// 160-bit big-endian window of five words.
+struct Win5 {
+    uint32_t w0, w1, w2, w3, w4;
+};
+
+// One of three words by i in {0, 1, 2}, with masks (kept as register
+// arithmetic; a select chain may be lowered to a private-memory table).
+__device__ __forceinline__ uint32_t sel3(int i, uint32_t a, uint32_t b, uint32_t c) {
+    const uint32_t m1 = 0u - (uint32_t)(i >= 1), m2 = 0u - (uint32_t)(i >= 2);
+    return (a & ~m1) | (b & m1 & ~m2) | (c & m2);
 }
 
-// Bits [o, o + 11) of the 96-bit big-endian window (a, b, c), o + 11 <= 96.
-__device__ __forceinline__ uint32_t win11(uint32_t a, uint32_t b, uint32_t c, int o) {
-    const uint64_t hi = ((uint64_t)a << 32) | b;
-    const uint64_t lo = ((uint64_t)b << 32) | c;
-    const uint64_t v = o < 32 ? hi : lo;
-    const int sh = o & 31;
-    return (uint32_t)((v << sh) >> 53);
+// Bits [o, o + 64) (0 <= o <= 95) of the window.
+__device__ __forceinline__ uint64_t win64(const Win5& w, int o) {
+    const int i = o >> 5, sh = o & 31;
+    const uint32_t x = sel3(i, w.w0, w.w1, w.w2);
+    const uint32_t y = sel3(i, w.w1, w.w2, w.w3);
+    const uint32_t z = sel3(i, w.w2, w.w3, w.w4);
+    const uint32_t hi = (uint32_t)(((((uint64_t)x << 32) | y) << sh) >> 32);
+    const uint32_t lo = (uint32_t)(((((uint64_t)y << 32) | z) << sh) >> 32);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t len_table(int brc) { return fdbaq_len_table(brc); }
+
+// Length of the symbol at the top of `hi` (sign bit, then the code).
+// tab: len_table(brc); x4 = 1 for BRC 4 (its '0x' codes are 2 or 3 bits).
+__device__ __forceinline__ int sym_len_hi(uint32_t hi, uint64_t tab, uint32_t x4) {
+    const int clo = min((int)__clz(~(hi << 1)), 10);
+    const int len = (int)((tab >> (4 * clo)) & 15u);
+    const uint32_t extra = clo == 0 ? ((hi >> 29) & x4) : 0u;
+    return 1 + len + (int)extra;
 }
 
 // Value of the lane to the left (lane 0 gets 0): DPP wave_shr:1.
@@ -111,40 +135,49 @@ __device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
     return x + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
 }
 
-__device__ __forceinline__ int sym_len(uint32_t a, uint32_t b, uint32_t c, int o, int brc) {
-    return 1 + fdbaq_code_len(brc, win11(a, b, c, o) & 0x3FFu);
-}
-
-// Position after `nsym` symbols of bit-rate code `brc` starting at bit
-// `start` (uniform across the wave).  Lane l owns the slice
-// [start + l*delta, start + (l+1)*delta) of a region of 64*delta bits:
-//   1. it decodes speculatively from its slice start, recording in `mask`
-//      the slice offsets where its symbols start, until it leaves the slice;
+// Position after `nsym` symbols of one BRC starting at bit `start`, found by
+// all 64 lanes.  Lane l owns the slice [start + l*delta, start + (l+1)*delta)
+// of a region of 64*delta bits:
+//   1. it decodes speculatively from kOverlap bits before its slice (or from
+//      `start`, a true boundary, if that is nearer), recording in `mask` the
+//      slice offsets where its symbols start, until it leaves the slice;
 //   2. its true entry is its left neighbour's exit; if that offset is one it
-//      already visited, its walk has synchronised and its exit stands,
-//      otherwise it walks again from the entry (a slice that a symbol from
-//      the left covers entirely passes the entry on as its exit); repeat
-//      until no exit changes;
+//      recorded, its exit stands, otherwise it walks again from the entry
+//      (a slice that a symbol from the left covers entirely passes the entry
+//      on as its exit); repeat until no exit changes;
 //   3. a prefix sum of per-lane symbol counts finds the lane holding the
-//      nsym-th symbol, which walks to its end.
+//      nsym-th symbol; the block ends where its next recorded symbol starts,
+//      or at its exit.
 // Regions repeat when one does not hold all nsym symbols.  delta <= 32.
-// Reads stay below word `wlim` (the staged words); once `start` passes
-// `nbits` the block cannot end inside the packet and the walk stops there.
-__device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, int64_t nbits, int64_t start, int brc,
-                              int nsym, int delta, int lane) {
+// *hdr receives the 8 bits that follow the end (the next block's BRC or
+// THIDX field).  Reads stay below word `wlim`; once `start` passes `nbits`
+// the block cannot end inside the packet and the walk stops there.
+__device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, int64_t nbits, int64_t start,
+                              uint64_t tab, uint32_t x4, int nsym, int delta, int lane, uint32_t* hdr) {
     for (;;) {
-        if (start > nbits) return start;
+        if (start > nbits) {
+            *hdr = 0;
+            return start;
+        }
         const int64_t s = start + (int64_t)lane * delta;
-        const int64_t w0 = s >> 5;
-        const uint32_t a = w0 < wlim ? words[w0] : 0u;
-        const uint32_t b = w0 + 1 < wlim ? words[w0 + 1] : 0u;
-        const uint32_t c = w0 + 2 < wlim ? words[w0 + 2] : 0u;
-        const int base = (int)(s & 31);
+        const int64_t org = max(s - kOverlap, start);
+        const int64_t w0 = org >> 5;
+        Win5 w;
+        w.w0 = w0 < wlim ? words[w0] : 0u;
+        w.w1 = w0 + 1 < wlim ? words[w0 + 1] : 0u;
+        w.w2 = w0 + 2 < wlim ? words[w0 + 2] : 0u;
+        w.w3 = w0 + 3 < wlim ? words[w0 + 3] : 0u;
+        w.w4 = w0 + 4 < wlim ? words[w0 + 4] : 0u;
+        // window offset of slice offset o is o + zero (zero <= 31 + kOverlap)
+        const int zero = (int)(org & 31) + (int)(s - org);
+        uint64_t win = win64(w, (int)(org & 31));
         uint32_t mask = 0;
-        int o = 0;
+        int o = (int)(org - s);
         while (o < delta) {
-            mask |= 1u << o;
-            o += sym_len(a, b, c, base + o, brc);
+            if (o >= 0) mask |= 1u << o;
+            const int l = sym_len_hi((uint32_t)(win >> 32), tab, x4);
+            o += l;
+            win <<= l;
         }
         int wex = o;  // exit of the recorded walk
         int ex = o;   // exit handed to the right neighbour
@@ -162,9 +195,12 @@ __device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, 
                 if (!((mask >> entry) & 1u)) {
                     mask = 0;
                     o = entry;
+                    uint64_t wr = win64(w, zero + entry);
                     while (o < delta) {
                         mask |= 1u << o;
-                        o += sym_len(a, b, c, base + o, brc);
+                        const int l = sym_len_hi((uint32_t)(wr >> 32), tab, x4);
+                        o += l;
+                        wr <<= l;
                     }
                     wex = o;
                     changed = true;
@@ -175,25 +211,40 @@ __device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, 
             ex = nex;
             if (__ballot(changed) == 0) break;
         }
-        const int cnt = entry >= delta ? 0 : __popc(mask >> entry);
+        const uint32_t live = entry >= delta ? 0u : (mask >> entry);
+        const int cnt = __popc(live);
         const int P = wave_inclusive_sum(cnt, lane);
         const int total = __builtin_amdgcn_readlane(P, 63);
         if (total >= nsym) {
             const unsigned long long bal = __ballot(P >= nsym);
             const int ls = __ffsll((long long)bal) - 1;
             int endrel = 0;
+            uint32_t h = 0;
             if (lane == ls) {
-                const int k = nsym - (P - cnt);
-                o = entry;
-                for (int i = 0; i < k; ++i) o += sym_len(a, b, c, base + o, brc);
-                endrel = o;
+                const int k = nsym - (P - cnt);  // block symbols starting in this slice, 1..cnt
+                if (k < cnt) {
+                    uint32_t m = live;
+                    for (int i = 0; i < k; ++i) m &= m - 1u;
+                    endrel = entry + __builtin_ctz(m);
+                } else {
+                    endrel = ex;
+                }
+                h = (uint32_t)(win64(w, zero + endrel) >> 56);
             }
             endrel = __builtin_amdgcn_readlane(endrel, ls);
+            *hdr = (uint32_t)__builtin_amdgcn_readlane((int)h, ls);
             return start + (int64_t)ls * delta + endrel;
         }
         nsym -= total;
         start += (int64_t)63 * delta + __builtin_amdgcn_readlane(ex, 63);
     }
+}
+
+// BRC of block b (uniform) from the lane-distributed registers.
+__device__ __forceinline__ int brc_of(int b, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) {
+    const int j = b >> 6, l = b & 63;
+    const uint32_t v = j == 0 ? v0 : j == 1 ? v1 : j == 2 ? v2 : v3;
+    return __builtin_amdgcn_readlane((int)v, l);
 }
 
 template <typename T>
@@ -210,15 +261,17 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
 
     uint32_t* words = smem;                                      // max_words + kPadWords
     int32_t* bstart = (int32_t*)(smem + max_words + kPadWords);  // 4 * max_blocks
-    uint8_t* bbrc = (uint8_t*)(bstart + 4 * max_blocks);         // max_blocks
-    uint8_t* bthr = bbrc + max_blocks;                           // max_blocks
+    uint8_t* bthr = (uint8_t*)(bstart + 4 * max_blocks);         // max_blocks
+    uint8_t* bbrc = bthr + max_blocks;                           // max_blocks
+    double* nrl = (double*)(((uintptr_t)(bbrc + max_blocks) + 7) & ~(uintptr_t)7);  // 5 * 16
+    for (int i = lane; i < 80; i += 64) nrl[i] = c_nrl[i >> 4][i & 15];
 
     const int nq = nqs[p];
     const int64_t nbytes = (int64_t)lens[p];
     const int nb = nq > 0 ? (nq + 127) / 128 : 0;
     T* orow = out + (int64_t)2 * ld_out * p;
     int st = S1G_PKT_OK;
-    if (nq < 0 || nb > max_blocks || (nbytes + 3) / 4 > max_words) st = S1G_PKT_ERR_ARG;
+    if (nq < 0 || nb > max_blocks || nb > kMaxBlocks || (nbytes + 3) / 4 > max_words) st = S1G_PKT_ERR_ARG;
 
     if (st == S1G_PKT_OK) {
         // ---- stage the user data as big-endian words, zero past the end
@@ -249,7 +302,11 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
     const int64_t nwords_staged = (nbytes + 3) / 4 + kPadWords;
     int64_t pos = 0;
     int prev_len = 0;
+    uint32_t vb0 = 0, vb1 = 0, vb2 = 0, vb3 = 0;  // BRC of block lane + 64 j
     for (int ch = 0; ch < 4 && st == S1G_PKT_OK; ++ch) {
+        // header field of the channel's first block, read from LDS; later
+        // ones come out of the walk of the block before
+        uint32_t hdr = (ch == 0 || ch == 2) ? peek_bits(words, pos, 8) : 0u;
         for (int b = 0; b < nb && st == S1G_PKT_OK; ++b) {
             int brc;
             if (ch == 0) {
@@ -257,24 +314,29 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
                     st = S1G_PKT_ERR_OVERRUN;
                     break;
                 }
-                const uint32_t v = peek_bits(words, pos, 3);
+                const uint32_t v = hdr >> 5;
                 if (v > 4) {
                     st = S1G_PKT_ERR_BRC;
                     break;
                 }
-                if (lane == 0) bbrc[b] = (uint8_t)v;
                 brc = (int)v;
+                const int j = b >> 6;
+                if (lane == (b & 63)) {
+                    if (j == 0) vb0 = v;
+                    else if (j == 1) vb1 = v;
+                    else if (j == 2) vb2 = v;
+                    else vb3 = v;
+                }
                 pos += 3;
             } else {
-                brc = bbrc[b];
+                brc = brc_of(b, vb0, vb1, vb2, vb3);
             }
             if (ch == 2) {
                 if (pos + 8 > nbits) {
                     st = S1G_PKT_ERR_OVERRUN;
                     break;
                 }
-                const uint32_t v = peek_bits(words, pos, 8);
-                if (lane == 0) bthr[b] = (uint8_t)v;
+                if (lane == 0) bthr[b] = (uint8_t)hdr;
                 pos += 8;
             }
             if (lane == 0) bstart[ch * max_blocks + b] = (int32_t)pos;
@@ -282,13 +344,18 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
             // slice width from the previous block of this channel (or a BRC guess)
             const int est = b ? prev_len : nsym * (brc + 3);
             const int delta = min(max((est + 63) / 64 + 1, 2), 32);
-            const int64_t end = walk_block(words, nwords_staged, nbits, pos, brc, nsym, delta, lane);
+            const int64_t end = walk_block(words, nwords_staged, nbits, pos, len_table(brc), brc == 4 ? 1u : 0u,
+                                           nsym, delta, lane, &hdr);
             prev_len = (int)(end - pos);
             pos = end;
             if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
         }
         pos = (pos + 15) & ~(int64_t)15;
-        __syncthreads();
+    }
+    // BRCs to LDS for phase 2 (bstart/bthr were written by lane 0)
+    for (int j = 0; j < 4; ++j) {
+        const int b = lane + 64 * j;
+        if (b < nb) bbrc[b] = (uint8_t)(j == 0 ? vb0 : j == 1 ? vb1 : j == 2 ? vb2 : vb3);
     }
     __syncthreads();
 
@@ -299,23 +366,47 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
         return;
     }
 
-    // ---- phase 2: lanes decode (channel, block) pairs
+    // ---- phase 2: lanes decode (channel, block) pairs from a register
+    // bit buffer and reconstruct (packet.cpp:678-693): for THIDX up to the
+    // BRC's limit, m below the flag gives m and m at the flag the simple
+    // level; otherwise NRL[brc][m] * SF[THIDX], one FP64 product.
     for (int item = lane; item < 4 * nb; item += 64) {
         const int ch = item & 3, b = item >> 2;
         const int brc = bbrc[b], th = bthr[b];
-        int64_t bp = bstart[ch * max_blocks + b];
+        const bool simple = th <= kThidxLimit[brc];
+        const int flag = kMFlag[brc];
+        const double sv = simple ? c_simple[brc][th] : 0.0;
+        const double sf = c_sf[th];
+        const double* nrow = nrl + 16 * brc;
+        const uint64_t tab = len_table(brc);
+        const uint32_t x4 = brc == 4 ? 1u : 0u;
+        const uint64_t mbase = kMBase[brc], mshift = kMShift[brc];
+        const uint32_t mmask = kMMask[brc];
+        const int64_t bp = bstart[ch * max_blocks + b];
+        int64_t wi = (bp >> 5) + 2;
+        uint64_t cur = (((uint64_t)words[wi - 2] << 32) | words[wi - 1]) << (bp & 31);
+        int valid = 64 - (int)(bp & 31);
+        uint32_t nxt = wi < nwords_staged ? words[wi] : 0u;
         const int q0 = 128 * b;
         const int cnt = (b == nb - 1) ? nq - q0 : 128;
         // out element 2q + (ch & 1), component ch >> 1 (IE: re even, IO: re odd, QE: im even, QO: im odd)
         T* dst = orow + 2 * (2 * (int64_t)q0 + (ch & 1)) + (ch >> 1);
         for (int s = 0; s < cnt; ++s) {
-            const uint32_t bits = peek_bits(words, bp, 11);
-            const uint32_t code = bits & 0x3FFu;
-            const int sign = (int)(bits >> 10);
-            const int clen = fdbaq_code_len(brc, code);
-            const int m = fdbaq_mcode(brc, code);
-            bp += 1 + clen;
-            dst[4 * (int64_t)s] = (T)recon(brc, th, sign, m);
+            if (valid < 32) {
+                cur |= (uint64_t)nxt << (32 - valid);
+                valid += 32;
+                ++wi;
+                nxt = wi < nwords_staged ? words[wi] : 0u;
+            }
+            const uint32_t v = (uint32_t)(cur >> 53);
+            const uint32_t code10 = v & 0x3FFu;
+            const int clo = fdbaq_clo(code10);
+            const int len = fdbaq_sym_len(code10, clo, tab, x4);
+            const int m = fdbaq_mcode_tab(code10, clo, mbase, mshift, mmask, x4);
+            const double mg = simple ? (m < flag ? (double)m : sv) : __dmul_rn(nrow[m], sf);
+            cur <<= len;
+            valid -= len;
+            dst[4 * (int64_t)s] = (T)((v >> 10) ? -mg : mg);
         }
     }
     if (lane == 0) status[p] = S1G_PKT_OK;
@@ -329,7 +420,8 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     const int max_blocks = (max_quads + 127) / 128 + 1;
     size_t lds = sizeof(uint32_t) * (size_t)(max_words + kPadWords) + sizeof(int32_t) * 4 * (size_t)max_blocks +
                  2 * (size_t)max_blocks;
-    lds = (lds + 15) & ~(size_t)15;
+    lds = (lds + 7) & ~(size_t)7;
+    lds += sizeof(double) * 80;
     if (lds > 160 * 1024) return -1;
     dim3 grid(n_packets), block(64);
     if (prec == S1G_F32)

@@ -11,7 +11,10 @@
 //   BRC4: 00 010 011 100 101 1100 1101 1110 11110 111110 11111100 11111101
 //         111111100 111111101 111111110 111111111       -> m 0..15
 // Each code is decided by its count of leading ones (clo) plus at most two
-// further bits, which the tables below encode.
+// further bits, which the tables below encode:
+//   BRC0-2: m = min(clo, flag);  BRC3: '0x' -> x, '10' -> 2, clo >= 2 -> min(clo + 1, 9);
+//   BRC4: '00' -> 0, '01x' -> 1 + x, '10x' -> 3 + x, '110x' -> 5 + x, clo 3..5 -> 7..9,
+//         '1111110x' -> 10 + x, '1111111xy' -> 12 + 2x + y.
 #pragma once
 
 #include <stdint.h>
@@ -47,38 +50,32 @@ S1G_HD int fdbaq_clo(uint32_t code10) {
 #endif
 }
 
-// Length in bits of the Huffman code at the top of `code10` (the 10 bits
-// after the sign bit).
-S1G_HD int fdbaq_code_len(int brc, uint32_t code10) {
-    const int clo = fdbaq_clo(code10);
-    const uint64_t t = brc == 0 ? kLen0 : brc == 1 ? kLen1 : brc == 2 ? kLen2 : brc == 3 ? kLen3 : kLen4;
-    int len = (int)((t >> (4 * clo)) & 15u);
-    if (brc == 4 && clo == 0) len += (int)((code10 >> 8) & 1u);
-    return len;
+// Branch-free rule used by the decoder.  Code length: 1 (sign) + table
+// entry by clo, plus the second code bit for BRC 4's '0x' codes.
+S1G_HD uint64_t fdbaq_len_table(int brc) {
+    return brc == 0 ? kLen0 : brc == 1 ? kLen1 : brc == 2 ? kLen2 : brc == 3 ? kLen3 : kLen4;
 }
-
-// Magnitude code m of the Huffman code at the top of `code10`.
-S1G_HD int fdbaq_mcode(int brc, uint32_t code10) {
-    const int clo = fdbaq_clo(code10);
-    const int b1 = (int)((code10 >> 8) & 1u), b2 = (int)((code10 >> 7) & 1u);
-    const int b3 = (int)((code10 >> 6) & 1u), b7 = (int)((code10 >> 2) & 1u), b8 = (int)((code10 >> 1) & 1u);
-    switch (brc) {
-        case 0: return clo < 3 ? clo : 3;
-        case 1: return clo < 4 ? clo : 4;
-        case 2: return clo < 6 ? clo : 6;
-        case 3: return clo == 0 ? b1 : clo == 1 ? 2 : (clo + 1 < 9 ? clo + 1 : 9);
-        default:
-            switch (clo) {
-                case 0: return b1 ? 1 + b2 : 0;
-                case 1: return 3 + b2;
-                case 2: return 5 + b3;
-                case 3: return 7;
-                case 4: return 8;
-                case 5: return 9;
-                case 6: return 10 + b7;
-                default: return 12 + 2 * b7 + b8;
-            }
-    }
+// v: sign bit followed by the 10 code bits; x4 = 1 for BRC 4.
+S1G_HD int fdbaq_sym_len(uint32_t code10, int clo, uint64_t len_tab, uint32_t x4) {
+    const int len = (int)((len_tab >> (4 * clo)) & 15u);
+    const uint32_t extra = clo == 0 ? ((code10 >> 8) & x4) : 0u;
+    return 1 + len + (int)extra;
+}
+// Magnitude code: m = base[clo] + ((code10 >> shift[clo]) & mask[clo]) with
+// 4-bit base and shift entries and 2-bit masks by clo; BRC 4 with clo 0
+// ('00' -> 0, '01x' -> 1 + x) is the one exception.
+constexpr uint64_t kMBase[5] = {0x33333333210ull, 0x44444443210ull, 0x66666543210ull, 0x99987654320ull,
+                                0xcccca987530ull};
+constexpr uint64_t kMShift[5] = {0ull, 0ull, 0ull, 0x8ull, 0x11112000670ull};
+constexpr uint32_t kMMask[5] = {0u, 0u, 0u, 0x1u, 0x3fd014u};
+S1G_HD int fdbaq_mcode_tab(uint32_t code10, int clo, uint64_t base, uint64_t shift, uint32_t mask, uint32_t x4) {
+    const int b = (int)((base >> (4 * clo)) & 15u);
+    const int sh = (int)((shift >> (4 * clo)) & 15u);
+    const int mk = (int)((mask >> (2 * clo)) & 3u);
+    int m = b + (int)((code10 >> sh) & (uint32_t)mk);
+    const int v2 = (int)((code10 >> 7) & 3u);
+    const int m4 = v2 >= 2 ? v2 - 1 : 0;
+    return (x4 && clo == 0) ? m4 : m;
 }
 
 // ----------------------------------------------------- reconstruction law
