@@ -1,16 +1,18 @@
 // azimuth.hip -- the range-Doppler azimuth chain.
 //
 //   azimuth FFT   column panels of W range samples x all L lines in LDS;
-//                 forward = digit-reversed load, DIT passes, fftshift on
-//                 store (s1_decoder.cpp:499-505); inverse = natural load, DIF
-//                 passes, digit-reversed gather on store, x 1/L
-//                 (s1_decoder.cpp:682, signal_processing.cpp:581-605).
+//                 natural-order load, DIF passes, digit-reversed gather on
+//                 store with the fftshift (forward, s1_decoder.cpp:499-505)
+//                 or the 1/L scale (inverse, s1_decoder.cpp:682,
+//                 signal_processing.cpp:581-605) folded in.
 //   RCMC + MF     one Doppler row tile per workgroup: FP64 migration offset
 //                 and phase per sample, 17-tap windowed-sinc interpolation
 //                 from an LDS tile, amplitude weight and matched filter
 //                 (formulas of s1_decoder.cpp:599-676, image_formation.cpp:
 //                 422-476, signal_processing.cpp:228-257).
 #include "s1g_internal.h"
+
+#include <initializer_list>
 
 namespace s1g {
 
@@ -21,8 +23,117 @@ __device__ __forceinline__ int xcd_panel(int bid, int npanels) {
     return (bid & 7) * per + (bid >> 3);
 }
 
+// In-place DIF passes with a compile-time radix sequence (DIT order
+// R0, R1, ...; DIF runs them last to first), so every pass is straight-line
+// code with its own register allocation.
+template <typename T, bool INV, int... Rs>
+struct DifSeq;
+
+template <typename T, bool INV>
+struct DifSeq<T, INV> {
+    static __device__ __forceinline__ void run(cpx<T>*, int, int, int, int, int, const cpx<T>*, int, int) {}
+};
+
+template <typename T, bool INV, int R, int... Rest>
+struct DifSeq<T, INV, R, Rest...> {
+    // Ns = product of the radices before R (those DIF applies after R)
+    static __device__ __forceinline__ void run(cpx<T>* buf, int n, int Ns, int nvec, int vstride, int estride,
+                                               const cpx<T>* tw, int tid, int nthr) {
+        DifSeq<T, INV, Rest...>::run(buf, n, Ns * R, nvec, vstride, estride, tw, tid, nthr);
+        dif_pass<T, R, INV>(buf, n, Ns, nvec, vstride, estride, tw, tid, nthr);
+        __syncthreads();
+    }
+};
+
+// One workgroup per panel of W adjacent range columns (W * sizeof(cpx<T>)
+// = 64 contiguous bytes per line), for a length whose radices are known at
+// compile time.  Every thread issues all its panel loads before the first
+// use (at most MAXE per thread); the twiddles and the digit-reversal map
+// are staged in LDS; natural-order input, DIF passes, and the store gathers
+// the digit-reversed output with the fftshift (forward) or the 1/L scale
+// (inverse) folded in.
+template <typename T, bool INV, int NT, int W, int MAXE, int... Rs>
+__global__ __launch_bounds__(NT) void az_fft_fixed_kernel(cpx<T>* __restrict__ data, int64_t ld, int N,
+                                                          int npanels, const cpx<T>* __restrict__ tw_g,
+                                                          const int32_t* __restrict__ perm_g, T scale) {
+    constexpr int L = (Rs * ...);
+    constexpr int total = L * W;
+    static_assert(total <= NT * MAXE, "panel fits the per-thread registers");
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = (cpx<T>*)smem_raw;        // L * W
+    cpx<T>* tw = buf + (size_t)L * W;       // L
+    int32_t* perm = (int32_t*)(tw + L);     // L
+    const int panel = xcd_panel(blockIdx.x, npanels);
+    if (panel >= npanels) return;
+    const int j0 = panel * W;
+    const int w = min(W, N - j0);
+    const int tid = threadIdx.x;
+    cpx<T> r[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int idx = tid + NT * i;
+        const int e = idx / W, v = idx % W;
+        r[i] = mk((T)0, (T)0);
+        if (idx < total && v < w) r[i] = data[(int64_t)e * ld + j0 + v];
+    }
+    for (int k = tid; k < L; k += NT) {
+        tw[k] = tw_g[k];
+        perm[k] = perm_g[k];
+    }
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int idx = tid + NT * i;
+        if (idx < total) buf[idx] = r[i];
+    }
+    __syncthreads();
+    DifSeq<T, INV, Rs...>::run(buf, L, 1, W, 1, W, tw, tid, NT);
+    // forward: out[i] = X[(i + L - L/2) mod L] (fftshift); inverse: x[k] / L
+    constexpr int sh = INV ? 0 : L - L / 2;
+#pragma unroll
+    for (int i = 0; i < MAXE; ++i) {
+        const int idx = tid + NT * i;
+        const int k = idx / W, v = idx % W;
+        if (idx < total && v < w) {
+            int src = k + sh;
+            if (src >= L) src -= L;
+            const cpx<T> y = buf[perm[src] * W + v];
+            data[(int64_t)k * ld + j0 + v] = INV ? scl(y, scale) : y;
+        }
+    }
+}
+
+template <typename T, bool INV, int W, int... Rs>
+static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale) {
+    constexpr int NT = 512;
+    constexpr int L = (Rs * ...);
+    constexpr int MAXE = (L * W + NT - 1) / NT;
+    const int npanels = (N + W - 1) / W;
+    const int per = (npanels + 7) / 8;
+    const size_t lds = (size_t)L * W * sizeof(cpx<T>) + (size_t)L * (sizeof(cpx<T>) + sizeof(int32_t));
+    hipLaunchKernelGGL((az_fft_fixed_kernel<T, INV, NT, W, MAXE, Rs...>), dim3((unsigned)(8 * per)), dim3(NT), lds,
+                       s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale);
+}
+
+// Plans with a compiled radix sequence (the IW1/IW2/IW3 burst lengths).
+template <typename T, bool INV, int W>
+static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale) {
+    const FftPlan& p = fl.plan;
+    auto is = [&](std::initializer_list<int> rs) {
+        if ((int)rs.size() != p.npass) return false;
+        int k = 0;
+        for (int r : rs)
+            if (p.radix[k++] != r) return false;
+        return true;
+    };
+    if (is({37, 5, 8})) launch_az_fixed<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale);
+    else if (is({71, 5, 4})) launch_az_fixed<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale);
+    else if (is({19, 5, 8, 2})) launch_az_fixed<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale);
+    else return false;
+    return true;
+}
+
 template <typename T, bool INV, int NT>
-__global__ __launch_bounds__(NT) void az_fft_kernel(cpx<T>* __restrict__ data, int64_t ld, int L, int N, int W,
+__global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__ data, int64_t ld, int L, int N, int W,
                                                     int npanels, FftPlan plan, const cpx<T>* __restrict__ tw,
                                                     const int32_t* __restrict__ perm, T scale) {
     extern __shared__ unsigned char smem_raw[];
@@ -65,8 +176,8 @@ __global__ __launch_bounds__(NT) void az_fft_kernel(cpx<T>* __restrict__ data, i
     }
 }
 
-int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
-                       int inverse) {
+static int launch_az_generic(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
+                             int inverse) {
     constexpr int NT = 512;
     // panel width: 64-byte row segments (8 complex64 / 4 complex128) within the LDS budget
     int W = prec == S1G_F32 ? 8 : 4;
@@ -80,21 +191,38 @@ int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, in
     if (prec == S1G_F32) {
         const float sc = (float)(1.0 / (double)L);
         if (inverse)
-            hipLaunchKernelGGL((az_fft_kernel<float, true, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L, N,
+            hipLaunchKernelGGL((az_fft_generic_kernel<float, true, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L, N,
                                W, npanels, fl.plan, fl.tw32, fl.perm, sc);
         else
-            hipLaunchKernelGGL((az_fft_kernel<float, false, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L,
+            hipLaunchKernelGGL((az_fft_generic_kernel<float, false, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L,
                                N, W, npanels, fl.plan, fl.tw32, fl.perm, 1.0f);
     } else {
         const double sc = 1.0 / (L);
         if (inverse)
-            hipLaunchKernelGGL((az_fft_kernel<double, true, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
+            hipLaunchKernelGGL((az_fft_generic_kernel<double, true, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
                                N, W, npanels, fl.plan, fl.tw64, fl.perm, sc);
         else
-            hipLaunchKernelGGL((az_fft_kernel<double, false, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
+            hipLaunchKernelGGL((az_fft_generic_kernel<double, false, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
                                N, W, npanels, fl.plan, fl.tw64, fl.perm, 1.0);
     }
     return (int)hipGetLastError();
+}
+
+int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
+                       int inverse) {
+    // panel width: 64-byte line segments (8 complex64 / 4 complex128)
+    bool done;
+    if (prec == S1G_F32) {
+        const float sc = (float)(1.0 / (double)L);
+        done = inverse ? az_fixed<float, true, 8>(s, fl, (cpx<float>*)data, ld, N, sc)
+                       : az_fixed<float, false, 8>(s, fl, (cpx<float>*)data, ld, N, 1.0f);
+    } else {
+        const double sc = 1.0 / (double)L;
+        done = inverse ? az_fixed<double, true, 4>(s, fl, (cpx<double>*)data, ld, N, sc)
+                       : az_fixed<double, false, 4>(s, fl, (cpx<double>*)data, ld, N, 1.0);
+    }
+    if (done) return (int)hipGetLastError();
+    return launch_az_generic(s, prec, fl, data, ld, L, N, inverse);
 }
 
 // ----------------------------------------------------------- column table

@@ -245,6 +245,21 @@ struct Dft<T, 16, INV> : DftCT<T, 4, 4, INV> {};
 template <typename T, bool INV>
 struct Dft<T, 32, INV> : DftCT<T, 8, 4, INV> {};
 
+// q = a / b, r = a % b for 0 <= a < 2^22, 1 <= b < 2^22, from a float
+// reciprocal: the product is within one of the quotient, then corrected.
+__device__ __forceinline__ int fast_div(int a, int b, float inv_b, int& r) {
+    int q = (int)((float)a * inv_b);
+    r = a - q * b;
+    if (r < 0) {
+        --q;
+        r += b;
+    } else if (r >= b) {
+        ++q;
+        r -= b;
+    }
+    return q;
+}
+
 // Radices the LDS passes support; the host planner factors lengths over them.
 #define S1G_RADIX_LIST(X) X(2) X(3) X(4) X(5) X(7) X(8) X(11) X(13) X(17) X(19) X(23) X(29) X(31) X(37) X(41) X(43) X(47) X(53) X(59) X(61) X(67) X(71)
 
@@ -262,10 +277,11 @@ __device__ __forceinline__ void dit_pass(cpx<T>* buf, int n, int Ns, int nvec, i
     const int nbf = n / R;
     const int total = nbf * nvec;
     const int tstep = n / (Ns * R);
+    const float inv_nvec = 1.0f / (float)nvec, inv_ns = 1.0f / (float)Ns;
     for (int idx = tid; idx < total; idx += nthr) {
-        const int v = idx % nvec;
-        const int bf = idx / nvec;
-        const int g = bf / Ns, j = bf - g * Ns;
+        int v, j;
+        const int bf = fast_div(idx, nvec, inv_nvec, v);
+        const int g = fast_div(bf, Ns, inv_ns, j);
         const int base = (g * Ns * R + j) * estride + v * vstride;
         cpx<T> a[R];
 #pragma unroll
@@ -293,10 +309,11 @@ __device__ __forceinline__ void dif_pass(cpx<T>* buf, int n, int Ns, int nvec, i
     const int nbf = n / R;
     const int total = nbf * nvec;
     const int tstep = n / (Ns * R);
+    const float inv_nvec = 1.0f / (float)nvec, inv_ns = 1.0f / (float)Ns;
     for (int idx = tid; idx < total; idx += nthr) {
-        const int v = idx % nvec;
-        const int bf = idx / nvec;
-        const int g = bf / Ns, j = bf - g * Ns;
+        int v, j;
+        const int bf = fast_div(idx, nvec, inv_nvec, v);
+        const int g = fast_div(bf, Ns, inv_ns, j);
         const int base = (g * Ns * R + j) * estride + v * vstride;
         cpx<T> a[R];
 #pragma unroll
