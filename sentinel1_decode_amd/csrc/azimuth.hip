@@ -231,6 +231,7 @@ __global__ void az_columns_kernel(s1g_azimuth_desc d, AzColumn* __restrict__ col
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d.n_samples) return;
     const double PI = 3.14159265358979;
+    const double inv2pi = 0.15915494309189535;  // 1 / (2 pi), true pi: phases in turns
     const double R = d.slant_ranges[j];
     const double v = d.v_rel[j];
     AzColumn c;
@@ -239,7 +240,10 @@ __global__ void az_columns_kernel(s1g_azimuth_desc d, AzColumn* __restrict__ col
     c.c = (2.0 * PI) * d.t_corr[j];
     c.rdr = R / d.range_spacing;
     c.fdc = d.f_dc[j];
-    c.amp = 1.0 / sqrt(pow(d.ref_range / R, 3.0));
+    c.sq = sqrt(pow(d.ref_range / R, 3.0));
+    c.amp = 1.0 / c.sq;
+    c.bt = c.b * inv2pi;
+    c.ct = c.c * inv2pi;
     cols[j] = c;
 }
 
@@ -266,73 +270,62 @@ constexpr double kSinK8[17] = {
     turn_sin(-2, 16), turn_sin(-1, 16), turn_sin(0, 16),  turn_sin(1, 16),  turn_sin(2, 16),  turn_sin(3, 16),
     turn_sin(4, 16),  turn_sin(5, 16),  turn_sin(6, 16),  turn_sin(7, 16),  turn_sin(8, 16)};
 
+// Per-sample quantities of the range-Doppler stage: the interpolation
+// position n0 + fr (t = j + (R/D - R)/dR, split so that fr keeps full
+// precision), the matched-filter phasor and the amplitude factor.
 template <typename T>
-struct SincWeights;
-
-template <>
-struct SincWeights<float> {
-    // sin(pi (fr - k)) = (-1)^k sinpi(fr); sinpi/cospi reduce their argument
-    // exactly, so the weights keep full relative accuracy for fr near 0 or 1.
-    static __device__ __forceinline__ void eval(float fr, float* w) {
-        const float pi = 3.14159265358979f;
-        float S, C, s8, c8;
-        sincospif(fr, &S, &C);
-        sincospif(fr * 0.125f, &s8, &c8);
-#pragma unroll
-        for (int k = -kTaps; k <= kTaps; ++k) {
-            const float x = fr - (float)k;
-            const float win = 0.5f * (1.0f + (c8 * (float)kCosK8[k + kTaps] + s8 * (float)kSinK8[k + kTaps]));
-            const float sgn = (k & 1) ? -1.0f : 1.0f;
-            const float sinc = (x == 0.0f) ? 1.0f : (sgn * S) / (pi * x);
-            w[k + kTaps] = sinc * win;
-        }
-    }
-};
-
-template <>
-struct SincWeights<double> {
-    // The reference evaluates sin(PI (fr - k)) per tap; the two nearest taps
-    // take exactly its arguments: PI fr (k = 0) and -PI (1 - fr) (k = 1).
-    // Even taps reuse sin(PI fr), odd taps -sin(PI (1 - fr)).
-    static __device__ __forceinline__ void eval(double fr, double* w) {
-#pragma clang fp contract(off)
-        const double PI = 3.14159265358979;
-        const double Se = sin(PI * fr);
-        const double So = sin(PI * (1.0 - fr));
-        double s8, c8;
-        sincos(PI * fr / kTaps, &s8, &c8);
-#pragma unroll
-        for (int k = -kTaps; k <= kTaps; ++k) {
-            const double x = fr - (double)k;
-            const double win = 0.5 * (1 + (c8 * kCosK8[k + kTaps] + s8 * kSinK8[k + kTaps]));
-            const double num = (k & 1) ? -So : Se;
-            const double sinc = (x == 0.0) ? 1.0 : num / (PI * x);
-            w[k + kTaps] = sinc * win;
-        }
-    }
-};
-
-// Per-sample geometry.  FP64 (reference expression order, used by the FP64
-// path): t = j + (R/D - R)/dR, phases ph1 = 4 PI R D / lambda and
-// ph2 = 2 PI (f + fdc) tc, amplitude 1/sqrt((Rref/R)^3).
 struct SampleGeo {
-    double t;
-    double c, s;  // e^{i (ph1 + ph2)}
-    double amp;
+    int n0;
+    T fr;
+    T c, s;
+    T amp;  // FP32: multiply; FP64: divide (the reference divides)
 };
 
-__device__ __forceinline__ SampleGeo geo_f64(const s1g_azimuth_desc& d, double fa, int j) {
+// FP32 chain.  FP64: f = f_a + f_dc, u = lambda^2 f^2 / (4 v^2),
+// D = sqrt(1 - u); the migration (R/D - R)/dR = rdr u / ((1 + D) D) avoids
+// the cancellation in 1/D - 1; the phase 4 PI R D / lambda + 2 PI f t_c is
+// formed in turns and reduced to [-1/2, 1/2] before the FP32 hardware
+// sin/cos (which take turns).
+__device__ __forceinline__ SampleGeo<float> geo32(const AzColumn& col, double fa, int j, bool rcmc) {
+    SampleGeo<float> g;
+    const double f = fa + col.fdc;
+    const double u = col.a * (f * f);
+    const double D = sqrt(1.0 - u);
+    const double den = (1.0 + D) * D;
+    double r = __builtin_amdgcn_rcp(den);
+    r = r * (2.0 - den * r);
+    const double shift = col.rdr * (u * r);
+    const double fl = floor(shift);
+    g.n0 = rcmc ? j + (int)fl : j;
+    g.fr = rcmc ? (float)(shift - fl) : 0.0f;
+    const double pt = col.bt * D + col.ct * f;
+    const float pr = (float)(pt - rint(pt));
+    g.s = __builtin_amdgcn_sinf(pr);
+    g.c = __builtin_amdgcn_cosf(pr);
+    g.amp = (float)col.amp;
+    return g;
+}
+
+// FP64 path: the reference's expressions and order (s1_decoder.cpp:599-676,
+// image_formation.cpp:439-441,469); the amplitude sqrt((Rref/R)^3) is the
+// per-column value of the same expression, and the phase 4 PI R D / lambda
+// (about 2e8 rad) is reduced mod 2 pi (three-part 2 pi) before sincos.
+__device__ __forceinline__ SampleGeo<double> geo64(const AzColumn& col, const s1g_azimuth_desc& d, double fa, int j,
+                                                   bool rcmc) {
 #pragma clang fp contract(off)
     const double PI = 3.14159265358979;
-    const double R = d.slant_ranges[j], v = d.v_rel[j], fdc = d.f_dc[j], tc = d.t_corr[j];
+    const double R = d.slant_ranges[j], v = d.v_rel[j], fdc = col.fdc, tc = d.t_corr[j];
     const double lam = d.wavelength;
     const double f = fa + fdc;
     const double D = sqrt(1 - ((lam * lam) * (f * f)) / (4 * v * v));
-    SampleGeo g;
+    SampleGeo<double> g;
     const double shift = (R / D) - R;
-    g.t = (double)j + shift / d.range_spacing;
-    g.amp = sqrt(pow(d.ref_range / R, 3.0));
-    const double ph1 = (((4.0 * PI) * R) * D) / lam;
+    const double t = (double)j + shift / d.range_spacing;
+    const double fl = floor(t);
+    g.n0 = rcmc ? (int)fl : j;
+    g.fr = rcmc ? t - fl : 0.0;
+    g.amp = col.sq;
+    const double ph1 = reduce_2pi((((4.0 * PI) * R) * D) / lam);
     const double ph2 = ((2.0 * PI) * (fa + fdc)) * tc;
     double s1, c1, s2, c2;
     sincos(ph1, &s1, &c1);
@@ -342,27 +335,48 @@ __device__ __forceinline__ SampleGeo geo_f64(const s1g_azimuth_desc& d, double f
     return g;
 }
 
-// FP32 chain: same quantities from the per-column FP64 table, the phase
-// reduced mod 2 pi in FP64 before the FP32 sincos.
-struct SampleGeo32 {
-    double t;
-    float c, s, amp;
-};
-
-__device__ __forceinline__ SampleGeo32 geo_f32(const AzColumn& col, double fa, int j) {
-    const double f = fa + col.fdc;
-    const double D = sqrt(1.0 - col.a * (f * f));
-    SampleGeo32 g;
-    g.t = (double)j + col.rdr * (1.0 / D - 1.0);
-    const double ph = reduce_2pi(col.b * D + col.c * f);
-    float sn, cs;
-    sincosf((float)ph, &sn, &cs);
-    g.c = cs;
-    g.s = sn;
-    g.amp = (float)col.amp;
-    return g;
+// w[k + 8] = sinc(fr - k) * 0.5 (1 + cos(PI (fr - k) / 8))
+//          = (-1)^k (sin(PI fr) / 2 PI) (1 + cos(PI fr/8) cos(PI k/8) + sin(PI fr/8) sin(PI k/8)) / (fr - k).
+// sinpi keeps full relative accuracy of sin(PI fr) for fr near 0 or 1.
+__device__ __forceinline__ void sinc_weights(float fr, float* w) {
+    const float S = sinpif(fr);
+    const float q = fr * 0.0625f;  // PI fr / 8 in turns
+    const float s8 = __builtin_amdgcn_sinf(q), c8 = __builtin_amdgcn_cosf(q);
+    const float A = S * 0.15915494309189535f;
+#pragma unroll
+    for (int k = -kTaps; k <= kTaps; ++k) {
+        const float num = 1.0f + (c8 * (float)kCosK8[k + kTaps] + s8 * (float)kSinK8[k + kTaps]);
+        const float v = A * num * __builtin_amdgcn_rcpf(fr - (float)k);
+        w[k + kTaps] = (k & 1) ? -v : v;
+    }
+    if (fr == 0.0f) w[kTaps] = 1.0f;
 }
 
+// FP64: the reference's per-tap expression sin(PI x)/(PI x) * window with
+// x = fr - k; the two nearest taps take exactly its arguments, PI fr
+// (k = 0) and -PI (1 - fr) (k = 1); even taps reuse sin(PI fr), odd taps
+// -sin(PI (1 - fr)).
+__device__ __forceinline__ void sinc_weights(double fr, double* w) {
+#pragma clang fp contract(off)
+    const double PI = 3.14159265358979;
+    const double Se = sin(PI * fr);
+    const double So = sin(PI * (1.0 - fr));
+    double s8, c8;
+    sincos(PI * fr / kTaps, &s8, &c8);
+#pragma unroll
+    for (int k = -kTaps; k <= kTaps; ++k) {
+        const double x = fr - (double)k;
+        const double win = 0.5 * (1 + (c8 * kCosK8[k + kTaps] + s8 * kSinK8[k + kTaps]));
+        const double num = (k & 1) ? -So : Se;
+        const double sinc = (x == 0.0) ? 1.0 : num / (PI * x);
+        w[k + kTaps] = sinc * win;
+    }
+}
+
+// One workgroup per (Doppler row i, run of NT * EPT range samples).  The
+// row segment the run's 17-tap windows reach is staged in LDS with zeros
+// outside [0, N), so the taps need no bounds tests; a run whose migration
+// span exceeds the tile falls back to bounds-checked global reads.
 template <typename T, int NT, int EPT>
 __global__ __launch_bounds__(NT) void rcmc_mf_kernel(s1g_azimuth_desc d, const AzColumn* __restrict__ cols,
                                                      const cpx<T>* __restrict__ in, int64_t ld_in,
@@ -374,6 +388,7 @@ __global__ __launch_bounds__(NT) void rcmc_mf_kernel(s1g_azimuth_desc d, const A
     const int i = blockIdx.y;
     const int tid = threadIdx.x;
     const int j0 = blockIdx.x * NT * EPT;
+    const bool rcmc = d.apply_rcmc != 0;
     // Doppler frequency of row i: linspace(-prf/2, prf/2, L)[i]
     double fa;
     {
@@ -383,47 +398,41 @@ __global__ __launch_bounds__(NT) void rcmc_mf_kernel(s1g_azimuth_desc d, const A
         fa = start + i * step;
     }
     const cpx<T>* row = in + (int64_t)i * ld_in;
-    double tt[EPT];
-    T cw[EPT], sw[EPT], aw[EPT];
+    SampleGeo<T> g[EPT];
     int lo = 1 << 30, hi = -(1 << 30);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const int j = j0 + e * NT + tid;
-        tt[e] = 0;
-        cw[e] = 0;
-        sw[e] = 0;
-        aw[e] = 0;
+        g[e].n0 = j;
+        g[e].fr = 0;
+        g[e].c = g[e].s = g[e].amp = 0;
         if (j < N) {
-            if (sizeof(T) == 8) {
-                SampleGeo g = geo_f64(d, fa, j);
-                tt[e] = d.apply_rcmc ? g.t : (double)j;
-                cw[e] = (T)g.c;
-                sw[e] = (T)g.s;
-                aw[e] = (T)g.amp;
-            } else {
-                SampleGeo32 g = geo_f32(cols[j], fa, j);
-                tt[e] = d.apply_rcmc ? g.t : (double)j;
-                cw[e] = (T)g.c;
-                sw[e] = (T)g.s;
-                aw[e] = (T)g.amp;
-            }
-            const int n0 = (int)floor(tt[e]);
-            lo = min(lo, n0 - kTaps);
-            hi = max(hi, n0 + kTaps);
+            if constexpr (sizeof(T) == 4)
+                g[e] = geo32(cols[j], fa, j, rcmc);
+            else
+                g[e] = geo64(cols[j], d, fa, j, rcmc);
+            lo = min(lo, g[e].n0 - kTaps);
+            hi = max(hi, g[e].n0 + kTaps);
         }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
     }
     if (tid == 0) {
         s_lo = 1 << 30;
         s_hi = -(1 << 30);
     }
     __syncthreads();
-    atomicMin(&s_lo, lo);
-    atomicMax(&s_hi, hi);
+    if ((tid & 63) == 0) {
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
+    }
     __syncthreads();
-    const int tlo = max(s_lo, 0), thi = min(s_hi, N - 1);
+    const int tlo = s_lo, thi = s_hi;
     const bool use_tile = (thi - tlo + 1) <= cap;
     if (use_tile)
-        for (int k = tlo + tid; k <= thi; k += NT) tile[k - tlo] = row[k];
+        for (int k = tlo + tid; k <= thi; k += NT) tile[k - tlo] = (k >= 0 && k < N) ? row[k] : mk((T)0, (T)0);
     __syncthreads();
     cpx<T>* orow = out + (int64_t)i * ld_out;
 #pragma unroll
@@ -431,28 +440,36 @@ __global__ __launch_bounds__(NT) void rcmc_mf_kernel(s1g_azimuth_desc d, const A
         const int j = j0 + e * NT + tid;
         if (j >= N) continue;
         cpx<T> acc = mk((T)0, (T)0);
-        if (d.apply_rcmc) {
-            const double t = tt[e];
-            const int n0 = (int)floor(t);
+        if (rcmc) {
             T w[2 * kTaps + 1];
-            SincWeights<T>::eval((T)(t - (double)n0), w);
+            sinc_weights(g[e].fr, w);
+            const int n0 = g[e].n0;
+            if (use_tile) {
+                const cpx<T>* tp = tile + (n0 - kTaps - tlo);
 #pragma unroll
-            for (int k = -kTaps; k <= kTaps; ++k) {
-                const int idx = n0 + k;
-                if (idx < 0 || idx >= N) continue;
-                const cpx<T> x = use_tile ? tile[idx - tlo] : row[idx];
-                acc.x += x.x * w[k + kTaps];
-                acc.y += x.y * w[k + kTaps];
+                for (int k = 0; k <= 2 * kTaps; ++k) {
+                    const cpx<T> x = tp[k];
+                    acc.x += x.x * w[k];
+                    acc.y += x.y * w[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k <= 2 * kTaps; ++k) {
+                    const int idx = n0 - kTaps + k;
+                    if (idx < 0 || idx >= N) continue;
+                    const cpx<T> x = row[idx];
+                    acc.x += x.x * w[k];
+                    acc.y += x.y * w[k];
+                }
             }
         } else {
             acc = row[j];
         }
-        if (sizeof(T) == 8) {
-            acc = mk(acc.x / aw[e], acc.y / aw[e]);
-        } else {
-            acc = scl(acc, aw[e]);
-        }
-        orow[j] = cmul(acc, mk(cw[e], sw[e]));
+        if constexpr (sizeof(T) == 8)
+            acc = mk(acc.x / g[e].amp, acc.y / g[e].amp);
+        else
+            acc = scl(acc, g[e].amp);
+        orow[j] = cmul(acc, mk(g[e].c, g[e].s));
     }
 }
 

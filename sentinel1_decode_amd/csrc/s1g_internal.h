@@ -27,7 +27,7 @@ __host__ __device__ inline const cpx<double>* tw_of<double>(const DevFft& f) { r
 template <>
 __host__ __device__ inline const cpx<float>* tw_of<float>(const DevFft& f) { return f.tw32; }
 
-// Per-column constants of the range-Doppler stage (FP32 chain).
+// Per-column constants of the range-Doppler stage.
 struct AzColumn {
     double a;     // lambda^2 / (4 v_rel^2)
     double b;     // 4 pi R_j / lambda
@@ -35,6 +35,9 @@ struct AzColumn {
     double rdr;   // R_j / range_spacing
     double fdc;   // Doppler centroid
     double amp;   // 1 / sqrt((R_ref / R_j)^3)
+    double sq;    // sqrt((R_ref / R_j)^3), the reference's divisor
+    double bt;    // b / (2 pi): phase coefficients in turns
+    double ct;    // c / (2 pi)
 };
 
 int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
