@@ -9,6 +9,7 @@ chain; the FP64 path (the reference arithmetic) is timed beside it, and the
 oracle (CPU FP64 restatement) is timed on the host cores as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py --copy-calibration   # plain copy for HBM counter scaling
 """
 from __future__ import annotations
 
@@ -39,6 +40,10 @@ def stage_bytes(prec_bytes, L, N, b_pkt):
 
 
 def run_gpu(burst, steps, warmup):
+    """Times the chain per precision three ways: eager launches and the
+    burst replayed as one captured hipGraph (both without stage timers; the
+    faster is the step time), then an eager pass with per-stage HIP-event
+    timers on the context stream for the stage breakdown."""
     import torch
     from sentinel1_decode_amd import capi, pipeline
 
@@ -50,26 +55,38 @@ def run_gpu(burst, steps, warmup):
     for name, prec, k in (("fp32", capi.S1G_F32, steps), ("fp64", capi.S1G_F64, max(2, steps // 2))):
         dt = torch.complex64 if prec == capi.S1G_F32 else torch.complex128
         out = torch.empty((L, N), dtype=dt, device="cuda")
+        desc = db.burst_desc(out)
         for _ in range(warmup):
             db.focus(ctx, prec, out=out)
+            ctx.focus_batch([desc], prec, graph=True)
         torch.cuda.synchronize()
+        times = {}
+        for mode in ("eager", "graph"):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                if mode == "eager":
+                    db.focus(ctx, prec, out=out)
+                else:
+                    ctx.focus_batch([desc], prec, graph=True)
+            torch.cuda.synchronize()
+            times[mode] = (time.perf_counter() - t0) * 1e3 / k
+        mode = min(times, key=times.get)
+        ms = times[mode]
         ctx.profile(True)
         ctx.profile_read(reset=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
         for _ in range(k):
             db.focus(ctx, prec, out=out)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
         stages = ctx.profile_read(reset=True)
         ctx.profile(False)
-        ms = (t1 - t0) * 1e3 / k
-        res[name] = {"ms_per_step": ms, "msamples_s": L * N / (ms * 1e-3) / 1e6, "steps": k,
+        res[name] = {"ms_per_step": ms, "msamples_s": L * N / (ms * 1e-3) / 1e6, "steps": k, "mode": mode,
+                     "ms_eager": times["eager"], "ms_graph": times["graph"],
                      "stages_ms": {s: v[0] / max(v[1], 1) for s, v in stages.items() if v[1]}}
         status = db.d_status.cpu().numpy()
         res[name]["packets_failed"] = int((status != 0).sum())
         outs[name] = out
-        print(f"# {name}: {ms:.3f} ms/step, {res[name]['msamples_s']:.1f} Msamples/s, stages "
+        print(f"# {name}: {ms:.3f} ms/step ({mode}; eager {times['eager']:.3f}, graph {times['graph']:.3f}), "
+              f"{res[name]['msamples_s']:.1f} Msamples/s, stages "
               + ", ".join(f"{s}={v:.3f}ms" for s, v in res[name]["stages_ms"].items()), file=sys.stderr, flush=True)
     a = outs["fp64"].cpu().numpy()
     b = outs["fp32"].cpu().numpy()
@@ -88,6 +105,24 @@ def run_gpu(burst, steps, warmup):
     return res, L, N
 
 
+def measured_traffic(kernel_prefix, L, N):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
+    plain copy), or None when the summary is for another shape."""
+    path = os.path.join(HERE, "profiles", "chain_iw2_burst_1480x24000_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if t["workload"] != {"lines": L, "samples": N}:
+        return None
+    for k, v in t["kernels"].items():
+        if k.startswith(kernel_prefix):
+            return round(v["read_bytes"] + v["write_bytes"])
+    return None
+
+
 def run_cpu_baseline(burst):
     """The oracle (CPU FP64 restatement) on the same packets, full chain."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
@@ -98,6 +133,21 @@ def run_cpu_baseline(burst):
     return out, t1 - t0
 
 
+def copy_calibration(mib=256, reps=5):
+    """A plain device copy of known size (torch elementwise kernel: read
+    mib MiB, write mib MiB per launch), the reference against which
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                                           "launches": reps}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,7 +156,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lines", type=int, default=1480)
     ap.add_argument("--samples", type=int, default=24000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain copy of known size (for counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); bursts shard across GPUs without "
               "communication, so the per-GPU figure is the single-GPU one.", file=sys.stderr)
@@ -140,6 +195,10 @@ def main():
     achieved = sb[dom] / (fp32["stages_ms"][dom] * 1e-3) / 1e9
     fft_frac = {s: stage_bytes(4, L, N, b_pkt)[s] / (fp32["stages_ms"][s] * 1e-3) / 1e9 / HBM_PEAK_GBS
                 for s in ("range", "az_fft", "az_ifft") if s in fp32["stages_ms"]}
+    kernel_of = {"decode": "fdbaq_decode_kernel<float>", "range": "range_os3_kernel<float",
+                 "az_fft": "az_fft_fixed_kernel<float, false", "rcmc_mf": "rcmc_mf_kernel<float",
+                 "az_ifft": "az_fft_fixed_kernel<float, true"}
+    traffic = measured_traffic(kernel_of.get(dom, "?"), L, N)
     line = {
         "metric": METRIC,
         "value": round(fp32["msamples_s"], 3),
@@ -156,11 +215,14 @@ def main():
         "config": {"workload": f"IW2 burst {L}x{N} FDBAQ packets -> SLC (400 Rayleigh targets, clutter 0.1, "
                                f"f_dc 30 Hz)", "lines": L, "samples": N, "packet_bytes": b_pkt},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": sb[dom]},
         "cpu_baseline": cpu,
         "fp64_msamples_s": round(fp64["msamples_s"], 3),
         "fp64_ms_per_step": round(fp64["ms_per_step"], 4),
         "fp32_over_fp64": round(fp32["msamples_s"] / fp64["msamples_s"], 3),
+        "mode": fp32["mode"],
+        "ms_eager_fp32": round(fp32["ms_eager"], 4), "ms_graph_fp32": round(fp32["ms_graph"], 4),
         "rel_l2_fp32_vs_fp64": res["rel_l2_fp32_vs_fp64"],
         "peak_phase_err_rad": res["peak_phase_err_rad"],
         "decode_mismatches": dec_mismatch,

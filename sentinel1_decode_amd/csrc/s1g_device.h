@@ -168,10 +168,42 @@ __device__ __forceinline__ void odd_out(cpx<T>* a, const cpx<T>* s, const cpx<T>
     }
 }
 
+// Same pairs, handed to sink(k, X_k) as soon as they are formed, so the R
+// outputs never need registers at the same time.
+template <typename T, int R, bool INV, int K, typename Sink>
+__device__ __forceinline__ void odd_out_sink(const cpx<T>* s, const cpx<T>* d, cpx<T> x0, Sink& sink) {
+    if constexpr (K <= (R - 1) / 2) {
+        T ar = x0.x, ai = x0.y, br = 0, bi = 0;
+        odd_acc<T, R, K, 1>(s, d, ar, ai, br, bi);
+        if (INV) {
+            sink(K, mk(ar - bi, ai + br));
+            sink(R - K, mk(ar + bi, ai - br));
+        } else {
+            sink(K, mk(ar + bi, ai - br));
+            sink(R - K, mk(ar - bi, ai + br));
+        }
+        odd_out_sink<T, R, INV, K + 1>(s, d, x0, sink);
+    }
+}
+
 template <typename T, int R, bool INV>
 struct Dft {
     static_assert(R % 2 == 1, "generic butterfly handles odd lengths");
     static constexpr int H = (R - 1) / 2;
+    template <typename Sink>
+    static __device__ __forceinline__ void run_sink(const cpx<T>* a, Sink& sink) {
+        cpx<T> s[H], d[H];
+        const cpx<T> x0 = a[0];
+        cpx<T> sum = a[0];
+#pragma unroll
+        for (int m = 1; m <= H; ++m) {
+            s[m - 1] = a[m] + a[R - m];
+            d[m - 1] = a[m] - a[R - m];
+            sum = sum + s[m - 1];
+        }
+        sink(0, sum);
+        odd_out_sink<T, R, INV, 1>(s, d, x0, sink);
+    }
     static __device__ __forceinline__ void run(cpx<T>* a) {
         cpx<T> s[H], d[H];
         const cpx<T> x0 = a[0];
@@ -293,9 +325,14 @@ __device__ __forceinline__ void dit_pass(cpx<T>* buf, int n, int Ns, int nvec, i
                 a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
             }
         }
-        Dft<T, R, INV>::run(a);
+        if constexpr (R % 2 == 1 && R >= 11) {
+            auto sink = [&](int k, cpx<T> v) { buf[base + k * Ns * estride] = v; };
+            Dft<T, R, INV>::run_sink(a, sink);
+        } else {
+            Dft<T, R, INV>::run(a);
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+            for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+        }
     }
 }
 
@@ -318,16 +355,27 @@ __device__ __forceinline__ void dif_pass(cpx<T>* buf, int n, int Ns, int nvec, i
         cpx<T> a[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = buf[base + r * Ns * estride];
-        Dft<T, R, INV>::run(a);
-        if (j) {
+        if constexpr (R % 2 == 1 && R >= 11) {
+            auto sink = [&](int k, cpx<T> v) {
+                if (k && j) {
+                    const cpx<T> w = tw[j * k * tstep];
+                    v = INV ? cmulc(v, w) : cmul(v, w);
+                }
+                buf[base + k * Ns * estride] = v;
+            };
+            Dft<T, R, INV>::run_sink(a, sink);
+        } else {
+            Dft<T, R, INV>::run(a);
+            if (j) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                cpx<T> w = tw[j * r * tstep];
-                a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+                for (int r = 1; r < R; ++r) {
+                    cpx<T> w = tw[j * r * tstep];
+                    a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+                }
             }
-        }
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+            for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+        }
     }
 }
 
