@@ -79,9 +79,11 @@ def run_gpu(burst, steps, warmup):
             db.focus(ctx, prec, out=out)
         stages = ctx.profile_read(reset=True)
         ctx.profile(False)
+        variants = {st: list(ctx.last_variant(st)) for st in ("range", "az_fft", "rcmc_mf", "az_ifft")}
         res[name] = {"ms_per_step": ms, "msamples_s": L * N / (ms * 1e-3) / 1e6, "steps": k, "mode": mode,
                      "ms_eager": times["eager"], "ms_graph": times["graph"],
-                     "stages_ms": {s: v[0] / max(v[1], 1) for s, v in stages.items() if v[1]}}
+                     "stages_ms": {s: v[0] / max(v[1], 1) for s, v in stages.items() if v[1]},
+                     "variants": variants}
         status = db.d_status.cpu().numpy()
         res[name]["packets_failed"] = int((status != 0).sum())
         outs[name] = out
@@ -230,6 +232,7 @@ def main():
         "stages_ms_fp32": {k: round(v, 4) for k, v in fp32["stages_ms"].items()},
         "stages_ms_fp64": {k: round(v, 4) for k, v in fp64["stages_ms"].items()},
         "hbm_frac_fft_passes_fp32": {k: round(v, 4) for k, v in fft_frac.items()},
+        "autotuned_variants": {"fp32": fp32["variants"], "fp64": fp64["variants"]},
     }
     print(json.dumps(line), flush=True)
 

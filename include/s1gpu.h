@@ -267,6 +267,23 @@ int s1g_af_adjoint(s1g_ctx* ctx, int prec, int n_lines, int n_samples, const voi
 int s1g_profile_enable(s1g_ctx* ctx, int on);
 int s1g_profile_read(s1g_ctx* ctx, double* ms, int64_t* launches, int reset);
 
+/* ----------------------------------------------------------- autotuner */
+/* (No reference counterpart.)  Stages with several launch variants --
+ * S1G_STAGE_RANGE (overlap-save block length), S1G_STAGE_AZ_FFT and
+ * S1G_STAGE_AZ_IFFT (panel width, compiled radix sequence or generic
+ * kernel), S1G_STAGE_RCMC_MF (workgroup tile) -- time every candidate on
+ * the first eager call for a shape (the in-place azimuth FFT on a scratch
+ * array), keep the fastest in the context, and reuse it; all variants give
+ * the same result to rounding.  Launches recorded into a graph use the
+ * cached choice; s1g_focus_batch runs the chain once eagerly before it
+ * captures.  Autotuning is on by default. */
+int s1g_set_autotune(s1g_ctx* ctx, int on);
+/* Use variant v (>= 0, clamped to the stage's count) for every later call of
+ * a stage, or return to the automatic choice with v = -1. */
+int s1g_force_variant(s1g_ctx* ctx, int stage, int variant);
+/* The variant the stage's last call used and how many it had to choose from. */
+int s1g_last_variant(const s1g_ctx* ctx, int stage, int* variant, int* n_variants);
+
 #ifdef __cplusplus
 }
 #endif

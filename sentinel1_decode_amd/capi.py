@@ -84,6 +84,9 @@ _SIGS = [
                                  C.POINTER(C.c_double), C.c_void_p]),
     ("s1g_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("s1g_profile_read", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("s1g_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
+    ("s1g_force_variant", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("s1g_last_variant", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
 STAGES = ["decode", "eccm", "range", "az_fft", "rcmc_mf", "az_ifft", "filter", "autofocus"]
@@ -178,6 +181,22 @@ class Context:
         if rc:
             msg = lib().s1g_last_error(self.h)
             raise S1GError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ------------------------------------------------------------- autotuner
+    def set_autotune(self, on=True):
+        self._check(lib().s1g_set_autotune(self.h, int(on)), "set_autotune")
+
+    def force_variant(self, stage, variant):
+        """Stage name (see STAGES) or index; variant -1 returns to automatic choice."""
+        st = STAGES.index(stage) if isinstance(stage, str) else int(stage)
+        self._check(lib().s1g_force_variant(self.h, st, int(variant)), "force_variant")
+
+    def last_variant(self, stage):
+        """(variant used by the stage's last call, number of variants it had)."""
+        st = STAGES.index(stage) if isinstance(stage, str) else int(stage)
+        v, n = C.c_int(0), C.c_int(0)
+        self._check(lib().s1g_last_variant(self.h, st, C.byref(v), C.byref(n)), "last_variant")
+        return v.value, n.value
 
     def profile(self, on=True):
         self._check(lib().s1g_profile_enable(self.h, int(on)), "profile_enable")

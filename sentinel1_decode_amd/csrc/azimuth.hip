@@ -114,6 +114,21 @@ static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64
                        s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale);
 }
 
+// Whether a plan's radix sequence is one compiled into az_fft_fixed_kernel.
+static int fixed_sequence(const FftPlan& p) {
+    auto is = [&](std::initializer_list<int> rs) {
+        if ((int)rs.size() != p.npass) return false;
+        int k = 0;
+        for (int r : rs)
+            if (p.radix[k++] != r) return false;
+        return true;
+    };
+    if (is({37, 5, 8})) return 1;
+    if (is({71, 5, 4})) return 2;
+    if (is({19, 5, 8, 2})) return 3;
+    return 0;
+}
+
 // Plans with a compiled radix sequence (the IW1/IW2/IW3 burst lengths).
 template <typename T, bool INV, int W>
 static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale) {
@@ -208,20 +223,28 @@ static int launch_az_generic(hipStream_t s, int prec, const DevFft& fl, void* da
     return (int)hipGetLastError();
 }
 
+int azimuth_fft_variants(const DevFft& fl) { return fixed_sequence(fl.plan) ? 3 : 1; }
+
+// Variants: with a compiled radix sequence 0 = 64-byte panels, 1 = 32-byte
+// panels (twice the workgroups per CU), 2 = the generic kernel; otherwise
+// only the generic kernel.
 int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
-                       int inverse) {
-    // panel width: 64-byte line segments (8 complex64 / 4 complex128)
-    bool done;
-    if (prec == S1G_F32) {
-        const float sc = (float)(1.0 / (double)L);
-        done = inverse ? az_fixed<float, true, 8>(s, fl, (cpx<float>*)data, ld, N, sc)
-                       : az_fixed<float, false, 8>(s, fl, (cpx<float>*)data, ld, N, 1.0f);
-    } else {
-        const double sc = 1.0 / (double)L;
-        done = inverse ? az_fixed<double, true, 4>(s, fl, (cpx<double>*)data, ld, N, sc)
-                       : az_fixed<double, false, 4>(s, fl, (cpx<double>*)data, ld, N, 1.0);
+                       int inverse, int variant) {
+    if (fixed_sequence(fl.plan) && variant < 2) {
+        const bool wide = variant == 0;
+        if (prec == S1G_F32) {
+            const float sc = (float)(1.0 / (double)L);
+            cpx<float>* d = (cpx<float>*)data;
+            if (wide) inverse ? az_fixed<float, true, 8>(s, fl, d, ld, N, sc) : az_fixed<float, false, 8>(s, fl, d, ld, N, 1.0f);
+            else inverse ? az_fixed<float, true, 4>(s, fl, d, ld, N, sc) : az_fixed<float, false, 4>(s, fl, d, ld, N, 1.0f);
+        } else {
+            const double sc = 1.0 / (double)L;
+            cpx<double>* d = (cpx<double>*)data;
+            if (wide) inverse ? az_fixed<double, true, 4>(s, fl, d, ld, N, sc) : az_fixed<double, false, 4>(s, fl, d, ld, N, 1.0);
+            else inverse ? az_fixed<double, true, 2>(s, fl, d, ld, N, sc) : az_fixed<double, false, 2>(s, fl, d, ld, N, 1.0);
+        }
+        return (int)hipGetLastError();
     }
-    if (done) return (int)hipGetLastError();
     return launch_az_generic(s, prec, fl, data, ld, L, N, inverse);
 }
 
@@ -473,18 +496,28 @@ __global__ __launch_bounds__(NT) void rcmc_mf_kernel(s1g_azimuth_desc d, const A
     }
 }
 
-int launch_rcmc_mf(hipStream_t s, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
-                   int64_t ld_in, void* out, int64_t ld_out) {
-    constexpr int NT = 256, EPT = 8;
+template <typename T, int NT, int EPT>
+static void launch_rcmc_t(hipStream_t s, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
+                          int64_t ld_in, void* out, int64_t ld_out) {
     const int per = NT * EPT;
     const int cap = per + 1024;
     dim3 grid((unsigned)((d.n_samples + per - 1) / per), (unsigned)d.n_lines);
-    if (prec == S1G_F32)
-        hipLaunchKernelGGL((rcmc_mf_kernel<float, NT, EPT>), grid, dim3(NT), sizeof(cpx<float>) * cap, s, d, cols,
-                           (const cpx<float>*)in, ld_in, (cpx<float>*)out, ld_out, cap);
-    else
-        hipLaunchKernelGGL((rcmc_mf_kernel<double, NT, EPT>), grid, dim3(NT), sizeof(cpx<double>) * cap, s, d,
-                           cols, (const cpx<double>*)in, ld_in, (cpx<double>*)out, ld_out, cap);
+    hipLaunchKernelGGL((rcmc_mf_kernel<T, NT, EPT>), grid, dim3(NT), sizeof(cpx<T>) * cap, s, d, cols,
+                       (const cpx<T>*)in, ld_in, (cpx<T>*)out, ld_out, cap);
+}
+
+// Variants (threads x samples per thread): 0 = 256 x 8, 1 = 256 x 4, 2 = 512 x 4.
+int launch_rcmc_mf(hipStream_t s, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
+                   int64_t ld_in, void* out, int64_t ld_out, int variant) {
+    if (prec == S1G_F32) {
+        if (variant == 1) launch_rcmc_t<float, 256, 4>(s, d, cols, in, ld_in, out, ld_out);
+        else if (variant == 2) launch_rcmc_t<float, 512, 4>(s, d, cols, in, ld_in, out, ld_out);
+        else launch_rcmc_t<float, 256, 8>(s, d, cols, in, ld_in, out, ld_out);
+    } else {
+        if (variant == 1) launch_rcmc_t<double, 256, 4>(s, d, cols, in, ld_in, out, ld_out);
+        else if (variant == 2) launch_rcmc_t<double, 512, 4>(s, d, cols, in, ld_in, out, ld_out);
+        else launch_rcmc_t<double, 256, 8>(s, d, cols, in, ld_in, out, ld_out);
+    }
     return (int)hipGetLastError();
 }
 

@@ -26,6 +26,15 @@ struct FilterKey {
     }
 };
 
+// Autotuner cache key: stage, precision and the shape parameters that
+// decide the stage's launch (lines, samples, replica length).
+struct TuneKey {
+    int stage, prec, a, b, c;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(stage, prec, a, b, c) < std::tie(o.stage, o.prec, o.a, o.b, o.c);
+    }
+};
+
 struct s1g_ctx {
     int device = 0;
     hipStream_t own = nullptr;
@@ -58,6 +67,12 @@ struct s1g_ctx {
     std::vector<Mark> marks;
     double stage_ms[S1G_NSTAGES] = {0};
     int64_t stage_n[S1G_NSTAGES] = {0};
+    // autotuner
+    bool autotune = true;
+    int force[S1G_NSTAGES] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    int last_var[S1G_NSTAGES] = {0};
+    int last_nvar[S1G_NSTAGES] = {0};
+    std::map<TuneKey, int> tuned;
 };
 
 // Brackets one stage's launches with events when profiling is on.
@@ -205,13 +220,72 @@ size_t esize(int prec) { return prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cp
 bool valid_prec(int prec) { return prec == S1G_F32 || prec == S1G_F64; }
 
 // Overlap-save FFT length: the smallest power of two holding a whole line
-// plus the chirp, else the largest one that fits 128 KB of LDS.
-int range_block_len(int prec, int n, int nc) {
-    const int mmax = prec == S1G_F32 ? 16384 : 8192;
+// plus the chirp, else the largest one up to mmax (at most what fits LDS:
+// 16384 complex64, 8192 complex128).
+int range_block_len(int prec, int n, int nc, int mmax = 0) {
+    if (mmax <= 0) mmax = prec == S1G_F32 ? 16384 : 8192;
     int m = 1;
     while (m < n + nc - 1 && m < mmax) m <<= 1;
     if (m <= nc) return -1;
     return m;
+}
+
+// Candidate overlap-save lengths (autotuner variants): the longest block
+// (fewest blocks per line) and, for FP32, the 8192 block that leaves room
+// for two workgroups per CU.
+int range_candidates(int prec, int n, int nc, int* cand) {
+    int k = 0;
+    const int caps[2] = {prec == S1G_F32 ? 16384 : 8192, 8192};
+    for (int cap : caps) {
+        const int m = range_block_len(prec, n, nc, cap);
+        if (m > 0 && (k == 0 || cand[k - 1] != m)) cand[k++] = m;
+    }
+    return k;
+}
+
+// Variant of a stage for this shape: forced, cached, or timed now.  run(v)
+// launches variant v on the context stream and returns 0 or an error; the
+// candidates are timed (best of two after a warm-up launch) only outside
+// graph capture, and the winner is cached.
+template <typename F>
+int choose_variant(s1g_ctx* c, const TuneKey& k, int nvar, F&& run) {
+    int v = 0;
+    if (c->force[k.stage] >= 0) {
+        v = std::min(c->force[k.stage], nvar - 1);
+    } else {
+        auto it = c->tuned.find(k);
+        if (it != c->tuned.end()) {
+            v = it->second;
+        } else if (c->autotune && !c->capturing && nvar > 1) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+                float best = 1e30f;
+                for (int cand = 0; cand < nvar; ++cand) {
+                    if (run(cand)) continue;
+                    float t = 1e30f;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        (void)hipEventRecord(a, c->stream);
+                        if (run(cand)) break;
+                        (void)hipEventRecord(b, c->stream);
+                        float ms = 0.f;
+                        if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess)
+                            t = std::min(t, ms);
+                    }
+                    if (t < best) {
+                        best = t;
+                        v = cand;
+                    }
+                }
+                c->tuned[k] = v;
+            }
+            if (a) hipEventDestroy(a);
+            if (b) hipEventDestroy(b);
+            c->err.clear();
+        }
+    }
+    c->last_var[k.stage] = v;
+    c->last_nvar[k.stage] = nvar;
+    return v;
 }
 
 int get_plan(s1g_ctx* c, int n, const DevFft** out) {
@@ -253,15 +327,30 @@ int do_range(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int6
     if (d.n_lines <= 0 || d.n_samples <= 0) return d.n_lines < 0 || d.n_samples < 0 ? fail(c, S1G_EARG, "bad range shape%s") : 0;
     const int nc = (int)floor(d.fs_mhz * d.txpl_us);
     if (nc < 2) return fail(c, S1G_EARG, "replica chirp shorter than 2 samples%s");
-    const int m = range_block_len(prec, d.n_samples, nc);
-    if (m < 0) return fail(c, S1G_EUNSUPPORTED, "replica chirp longer than the overlap-save block%s");
+    int cand[2];
+    const int nvar = range_candidates(prec, d.n_samples, nc, cand);
+    if (nvar == 0) return fail(c, S1G_EUNSUPPORTED, "replica chirp longer than the overlap-save block%s");
+    auto prepare = [&](int v, const DevFft** fm, void** h) {
+        int rc = get_plan(c, cand[v], fm);
+        if (!rc) rc = get_filter(c, d, prec, nc, *fm, h);
+        return rc;
+    };
+    auto run = [&](int v) {
+        const DevFft* fm;
+        void* h;
+        int rc = prepare(v, &fm, &h);
+        if (rc) return rc;
+        return check_launch(c, launch_range_os(c->stream, prec, *fm, in, ld_in, out, ld_out, d.n_lines, d.n_samples,
+                                               nc, h),
+                            "range compression");
+    };
+    const int v = choose_variant(c, TuneKey{S1G_STAGE_RANGE, prec, d.n_lines, d.n_samples, nc}, nvar, run);
     const DevFft* fm;
-    int rc = get_plan(c, m, &fm);
-    if (rc) return rc;
     void* h;
+    int rc;
     {
         StageTimer tm(c, S1G_STAGE_FILTER);
-        rc = get_filter(c, d, prec, nc, fm, &h);
+        rc = prepare(v, &fm, &h);
     }
     if (rc) return rc;
     StageTimer tm(c, S1G_STAGE_RANGE);
@@ -274,8 +363,26 @@ int do_azimuth_fft(s1g_ctx* c, int prec, int L, int N, void* data, int64_t ld, i
     const DevFft* fl;
     int rc = get_plan(c, L, &fl);
     if (rc) return rc;
-    StageTimer tm(c, inverse ? S1G_STAGE_AZ_IFFT : S1G_STAGE_AZ_FFT);
-    return check_launch(c, launch_azimuth_fft(c->stream, prec, *fl, data, ld, L, N, inverse), "azimuth fft");
+    const int stage = inverse ? S1G_STAGE_AZ_IFFT : S1G_STAGE_AZ_FFT;
+    // the transform is in place: candidates are timed on a scratch array
+    void* tmp = nullptr;
+    auto run = [&](int v) {
+        if (!tmp) {
+            if (hipMalloc(&tmp, esize(prec) * (size_t)L * N) != hipSuccess) {
+                tmp = nullptr;
+                return (int)S1G_ENOMEM;
+            }
+            (void)hipMemsetAsync(tmp, 0, esize(prec) * (size_t)L * N, c->stream);
+        }
+        return check_launch(c, launch_azimuth_fft(c->stream, prec, *fl, tmp, N, L, N, inverse, v), "azimuth fft");
+    };
+    const int v = choose_variant(c, TuneKey{stage, prec, L, N, 0}, azimuth_fft_variants(*fl), run);
+    if (tmp) {
+        (void)hipStreamSynchronize(c->stream);
+        hipFree(tmp);
+    }
+    StageTimer tm(c, stage);
+    return check_launch(c, launch_azimuth_fft(c->stream, prec, *fl, data, ld, L, N, inverse, v), "azimuth fft");
 }
 
 int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int64_t ld_in, void* out,
@@ -284,10 +391,16 @@ int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int
     if (!d.slant_ranges || !d.v_rel || !d.f_dc || !d.t_corr) return fail(c, S1G_EARG, "null geometry array%s");
     int rc = ensure(c, (void**)&c->cols, &c->cols_n, sizeof(AzColumn) * (size_t)d.n_samples);
     if (rc) return rc;
-    StageTimer tm(c, S1G_STAGE_RCMC_MF);
     rc = launch_az_columns(c->stream, d, c->cols);
     if (rc) return hip_fail(c, (hipError_t)rc, "azimuth columns");
-    return check_launch(c, launch_rcmc_mf(c->stream, prec, d, c->cols, in, ld_in, out, ld_out), "rcmc/matched filter");
+    auto run = [&](int v) {
+        return check_launch(c, launch_rcmc_mf(c->stream, prec, d, c->cols, in, ld_in, out, ld_out, v),
+                            "rcmc/matched filter");
+    };
+    const int v = choose_variant(c, TuneKey{S1G_STAGE_RCMC_MF, prec, d.n_lines, d.n_samples, d.apply_rcmc},
+                                 kRcmcVariants, run);
+    StageTimer tm(c, S1G_STAGE_RCMC_MF);
+    return run(v);
 }
 
 int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_t ld_in, void* out, int64_t ld_out,
@@ -391,6 +504,25 @@ int s1g_synchronize(s1g_ctx* c) {
 int s1g_profile_enable(s1g_ctx* c, int on) {
     if (!c) return S1G_EARG;
     c->profiling = on != 0;
+    return S1G_OK;
+}
+
+int s1g_set_autotune(s1g_ctx* c, int on) {
+    if (!c) return S1G_EARG;
+    c->autotune = on != 0;
+    return S1G_OK;
+}
+
+int s1g_force_variant(s1g_ctx* c, int stage, int variant) {
+    if (!c || stage < 0 || stage >= S1G_NSTAGES || variant < -1) return S1G_EARG;
+    c->force[stage] = variant;
+    return S1G_OK;
+}
+
+int s1g_last_variant(const s1g_ctx* c, int stage, int* variant, int* n_variants) {
+    if (!c || stage < 0 || stage >= S1G_NSTAGES) return S1G_EARG;
+    if (variant) *variant = c->last_var[stage];
+    if (n_variants) *n_variants = c->last_nvar[stage];
     return S1G_OK;
 }
 
@@ -622,13 +754,16 @@ static int prepare_burst(s1g_ctx* c, const s1g_burst_desc& b, int prec, size_t* 
     if (L <= 1 || N <= 0) return fail(c, S1G_EARG, "burst needs >= 2 lines%s");
     const int nc = (int)floor(b.range.fs_mhz * b.range.txpl_us);
     if (nc < 2) return fail(c, S1G_EARG, "replica chirp shorter than 2 samples%s");
-    const int m = range_block_len(prec, N, nc);
-    if (m < 0) return fail(c, S1G_EUNSUPPORTED, "replica chirp longer than the overlap-save block%s");
+    int cand[2];
+    const int nvar = range_candidates(prec, N, nc, cand);
+    if (nvar == 0) return fail(c, S1G_EUNSUPPORTED, "replica chirp longer than the overlap-save block%s");
     const DevFft *fm, *fl;
-    int rc = get_plan(c, m, &fm);
-    if (!rc) rc = get_plan(c, L, &fl);
-    void* h;
-    if (!rc) rc = get_filter(c, b.range, prec, nc, fm, &h);
+    int rc = get_plan(c, L, &fl);
+    for (int v = 0; v < nvar && !rc; ++v) {
+        rc = get_plan(c, cand[v], &fm);
+        void* h;
+        if (!rc) rc = get_filter(c, b.range, prec, nc, fm, &h);
+    }
     if (!rc && b.use_eccm) {
         const DevFft* f;
         rc = get_fft(c, N, &f);
@@ -667,11 +802,16 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
         }
         return S1G_OK;
     }
-    // one hipGraph per distinct descriptor set
+    // one hipGraph per distinct descriptor set (and forced variants)
     std::string key((const char*)descs, sizeof(s1g_burst_desc) * (size_t)n);
     key.append((const char*)&prec, sizeof prec);
+    key.append((const char*)c->force, sizeof c->force);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
+        // one eager pass first: it picks (and caches) each stage's launch
+        // variant and builds everything the captured launches use
+        for (int i = 0; i < n && !rc; ++i) rc = burst_chain(c, descs[i], prec);
+        if (rc) return rc;
         hipStream_t user = c->stream;
         HIPCHK(c, hipStreamSynchronize(user));  // filters built above are complete
         c->stream = c->own;

@@ -57,11 +57,15 @@ int launch_range_os(hipStream_t stream, int prec, const DevFft& fm, const void* 
                     int64_t ld_out, int n_lines, int n_samples, int nc, const void* h);
 
 // Azimuth stages.
+// Launch variants (candidates of the autotuner): variant counts and the
+// launch with a given variant.
+int azimuth_fft_variants(const DevFft& fl);
+constexpr int kRcmcVariants = 3;
 int launch_azimuth_fft(hipStream_t stream, int prec, const DevFft& fl, void* data, int64_t ld, int n_lines,
-                       int n_samples, int inverse);
+                       int n_samples, int inverse, int variant);
 int launch_az_columns(hipStream_t stream, const s1g_azimuth_desc& d, AzColumn* cols);
 int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
-                   int64_t ld_in, void* out, int64_t ld_out);
+                   int64_t ld_in, void* out, int64_t ld_out, int variant);
 
 // ECCM.
 // Out of place (windows overlap, so workgroups would race in place).

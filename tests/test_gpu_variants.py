@@ -1,0 +1,123 @@
+"""Every launch variant the autotuner can pick gives the oracle's result.
+
+Each tunable stage is run with each of its variants forced
+(s1g_force_variant) and compared with the CPU oracle at the stage's
+tolerance; then the automatic choice is checked to be one of the
+candidates, and a graph captured with a forced variant to replay the eager
+result.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from chain_ref import oracle_chain
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+from sentinel1_decode_amd import capi, pipeline, synth  # noqa: E402
+
+F32, F64 = capi.S1G_F32, capi.S1G_F64
+
+
+@pytest.fixture
+def vctx():
+    c = capi.Context(0)
+    yield c
+    c.close()
+
+
+def _variant_count(ctx, stage, run):
+    ctx.force_variant(stage, 0)
+    run()
+    torch.cuda.synchronize()
+    return ctx.last_variant(stage)[1]
+
+
+@pytest.mark.parametrize("prec,tol", [(F64, 1e-11), (F32, 2e-6)])
+def test_range_variants(vctx, prec, tol):
+    rng = np.random.default_rng(5)
+    N = 24000
+    x = (rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))) * 20
+    txpsf, txprr, txpl, fs = -8.7e-1, 1.0789, 52.40481, 64.34523813
+    ref = oracle.range_compress(x, txpsf, txprr, txpl, fs)
+    dt = torch.complex128 if prec == F64 else torch.complex64
+    d = torch.from_numpy(x).to(dt).cuda()
+    desc = capi.RangeDesc(3, N, txpsf, txprr, txpl, fs)
+    n = _variant_count(vctx, "range", lambda: vctx.range_compress(d, desc))
+    assert n == (2 if prec == F32 else 1)
+    for v in range(n):
+        vctx.force_variant("range", v)
+        y = vctx.range_compress(d, desc)
+        torch.cuda.synchronize()
+        assert vctx.last_variant("range") == (v, n)
+        assert rel_l2(y.cpu().numpy(), ref) < tol, v
+
+
+@pytest.mark.parametrize("L", [1480, 1420, 1520])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_azimuth_variants(vctx, L, inverse):
+    rng = np.random.default_rng(L + inverse)
+    a = rng.standard_normal((L, 100)) + 1j * rng.standard_normal((L, 100))
+    ref = oracle.azimuth_ifft(a) if inverse else oracle.azimuth_fft(a)
+    stage = "az_ifft" if inverse else "az_fft"
+    for prec, dt, tol in ((F64, torch.complex128, 1e-13), (F32, torch.complex64, 2e-6)):
+        x0 = torch.from_numpy(a).to(dt).cuda()
+        n = _variant_count(vctx, stage, lambda: vctx.azimuth_fft(x0.clone(), inverse=inverse))
+        assert n == 3
+        for v in range(n):
+            vctx.force_variant(stage, v)
+            x = x0.clone()
+            vctx.azimuth_fft(x, inverse=inverse)
+            torch.cuda.synchronize()
+            assert vctx.last_variant(stage) == (v, n)
+            assert rel_l2(x.cpu().numpy(), ref) < tol, (prec, v)
+
+
+@pytest.fixture(scope="module")
+def rd_case():
+    spec = synth.config0(n_lines=192, n_samples=4096, grid_spacing=(300, 40), seed=7)
+    burst = synth.make_burst(spec)
+    return burst, oracle_chain(burst)
+
+
+@pytest.mark.parametrize("prec,tol", [(F64, 1e-11), (F32, 2e-6)])
+def test_rcmc_variants(vctx, rd_case, prec, tol):
+    burst, ch = rd_case
+    db = pipeline.from_synth(burst)
+    dt = torch.complex128 if prec == F64 else torch.complex64
+    x = torch.from_numpy(ch["rd"]).to(dt).cuda()
+    n = _variant_count(vctx, "rcmc_mf", lambda: vctx.rcmc_matched_filter(x, db.azimuth_desc()))
+    assert n == 3
+    for v in range(n):
+        vctx.force_variant("rcmc_mf", v)
+        y = vctx.rcmc_matched_filter(x, db.azimuth_desc())
+        torch.cuda.synchronize()
+        assert vctx.last_variant("rcmc_mf") == (v, n)
+        assert rel_l2(y.cpu().numpy(), ch["mf"]) < tol, v
+
+
+def test_autotuned_chain_and_forced_graph(vctx):
+    """Automatic choice stays among the candidates and matches a forced
+    run; a graph captured under forced variants replays the eager result."""
+    spec = synth.config1(n_lines=256, n_samples=24000)
+    burst = synth.make_burst(spec)
+    db = pipeline.from_synth(burst)
+    auto = db.focus(vctx, F32)
+    torch.cuda.synchronize()
+    for st in ("range", "az_fft", "rcmc_mf", "az_ifft"):
+        v, n = vctx.last_variant(st)
+        assert 0 <= v < n, st
+    for st in ("range", "az_fft", "rcmc_mf", "az_ifft"):
+        vctx.force_variant(st, 1)
+    forced = db.focus(vctx, F32)
+    torch.cuda.synchronize()
+    assert rel_l2(forced.cpu().numpy(), auto.cpu().numpy()) < 1e-6
+    out = torch.empty_like(forced)
+    desc = db.burst_desc(out)
+    for rep in range(3):
+        out.zero_()
+        vctx.focus_batch([desc], F32, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(out, forced), rep
