@@ -109,12 +109,27 @@ __device__ __forceinline__ uint64_t win64(const Win5& w, int o) {
 __device__ __forceinline__ uint64_t len_table(int brc) { return fdbaq_len_table(brc); }
 
 // Length of the symbol at the top of `hi` (sign bit, then the code).
-// tab: len_table(brc); x4 = 1 for BRC 4 (its '0x' codes are 2 or 3 bits).
-__device__ __forceinline__ int sym_len_hi(uint32_t hi, uint64_t tab, uint32_t x4) {
-    const int clo = min((int)__clz(~(hi << 1)), 10);
-    const int len = (int)((tab >> (4 * clo)) & 15u);
-    const uint32_t extra = clo == 0 ? ((hi >> 29) & x4) : 0u;
-    return 1 + len + (int)extra;
+// BRC 0-3: the code length is med3(clo + 1, lo, hi) with (lo, hi) = (1, 3),
+// (1, 4), (1, 6), (2, 8), so the symbol length is med3(clo + 2, lo + 1,
+// hi + 1) (lo1, hi1 below).  BRC 4 reads its nibble table `tab` and adds
+// the second code bit for its '0x' codes.
+template <bool B4>
+__device__ __forceinline__ int sym_len_hi(uint32_t hi, uint64_t tab, int lo1, int hi1) {
+    const int clo = (int)__clz(~(hi << 1));
+    if constexpr (!B4) {
+        return min(max(clo + 2, lo1), hi1);
+    } else {
+        const int c = min(clo, 10);
+        const int len = (int)((tab >> (4 * c)) & 15u);
+        const uint32_t extra = c == 0 ? ((hi >> 29) & 1u) : 0u;
+        return 1 + len + (int)extra;
+    }
+}
+
+// (lo + 1, hi + 1) of the BRC 0-3 code-length rule.
+__device__ __forceinline__ void len_bounds(int brc, int& lo1, int& hi1) {
+    lo1 = brc == 3 ? 3 : 2;
+    hi1 = brc == 0 ? 4 : brc == 1 ? 5 : brc == 2 ? 7 : 9;
 }
 
 // Value of the lane to the left (lane 0 gets 0): DPP wave_shr:1.
@@ -152,8 +167,9 @@ __device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
 // *hdr receives the 8 bits that follow the end (the next block's BRC or
 // THIDX field).  Reads stay below word `wlim`; once `start` passes `nbits`
 // the block cannot end inside the packet and the walk stops there.
+template <bool B4>
 __device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, int64_t nbits, int64_t start,
-                              uint64_t tab, uint32_t x4, int nsym, int delta, int lane, uint32_t* hdr) {
+                              uint64_t tab, int lo1, int hi1, int nsym, int delta, int lane, uint32_t* hdr) {
     for (;;) {
         if (start > nbits) {
             *hdr = 0;
@@ -173,9 +189,15 @@ __device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, 
         uint64_t win = win64(w, (int)(org & 31));
         uint32_t mask = 0;
         int o = (int)(org - s);
+        // lead-in: the overlap before the slice (nothing recorded)
+        while (o < 0) {
+            const int l = sym_len_hi<B4>((uint32_t)(win >> 32), tab, lo1, hi1);
+            o += l;
+            win <<= l;
+        }
         while (o < delta) {
-            if (o >= 0) mask |= 1u << o;
-            const int l = sym_len_hi((uint32_t)(win >> 32), tab, x4);
+            mask |= 1u << o;
+            const int l = sym_len_hi<B4>((uint32_t)(win >> 32), tab, lo1, hi1);
             o += l;
             win <<= l;
         }
@@ -198,7 +220,7 @@ __device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, 
                     uint64_t wr = win64(w, zero + entry);
                     while (o < delta) {
                         mask |= 1u << o;
-                        const int l = sym_len_hi((uint32_t)(wr >> 32), tab, x4);
+                        const int l = sym_len_hi<B4>((uint32_t)(wr >> 32), tab, lo1, hi1);
                         o += l;
                         wr <<= l;
                     }
@@ -344,8 +366,11 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
             // slice width from the previous block of this channel (or a BRC guess)
             const int est = b ? prev_len : nsym * (brc + 3);
             const int delta = min(max((est + 63) / 64 + 1, 2), 32);
-            const int64_t end = walk_block(words, nwords_staged, nbits, pos, len_table(brc), brc == 4 ? 1u : 0u,
-                                           nsym, delta, lane, &hdr);
+            int lo1, hi1;
+            len_bounds(brc, lo1, hi1);
+            const int64_t end =
+                brc == 4 ? walk_block<true>(words, nwords_staged, nbits, pos, kLen4, lo1, hi1, nsym, delta, lane, &hdr)
+                         : walk_block<false>(words, nwords_staged, nbits, pos, 0, lo1, hi1, nsym, delta, lane, &hdr);
             prev_len = (int)(end - pos);
             pos = end;
             if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
