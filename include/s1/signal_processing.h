@@ -1,0 +1,39 @@
+/*
+ * signal_processing.h -- drop-in building blocks of the reference's
+ * signal_processing.h used on the L0 -> SLC path, computed on the GPU
+ * through the C ABI (FP64).  Signatures follow signal_processing.h:15-180.
+ * get_fftw_plans / destroy_fftw_plans expose FFTW types and have no
+ * counterpart: plans live in the s1g context.
+ */
+#pragma once
+
+#include "misc_types.h"
+
+/* linspace, endpoint included (signal_processing.cpp:18-29). */
+D_VEC_1D linspace(const double& start, const double& end, const int& size);
+
+/* Hann-windowed sinc interpolation at fractional index t over 2 L + 1
+ * samples (signal_processing.cpp:238-257). */
+std::complex<double> sinc_interpolate(const CF_VEC_1D& signal, double t, int L = 8);
+
+CF_VEC_2D transpose(const CF_VEC_2D& arr);
+
+/* DFTs with FFTW's conventions: forward unnormalised, inverse x 1/fft_size
+ * (signal_processing.cpp:516-681).  fft_size = 0 (or <= 0 for the 1-D
+ * forms) means the full length; a shorter fft_size transforms the leading
+ * fft_size samples of each row. */
+CF_VEC_1D compute_1d_dft(const CF_VEC_1D& signal, int fft_size = 0, const bool& inverse = false);
+void compute_1d_dft_in_place(CF_VEC_1D& signal, int fft_size = 0, const bool& inverse = false);
+void _compute_axis_dft(CF_VEC_2D& signals, int& fft_size, const bool& inverse);
+CF_VEC_2D compute_axis_dft(CF_VEC_2D& signals, int fft_size = 0, const int& axis = 0, const bool& inverse = false);
+void compute_axis_dft_in_place(CF_VEC_2D& signals, int fft_size = 0, const int& axis = 0,
+                               const bool& inverse = false);
+
+/* std::rotate by size/2 (signal_processing.cpp:713-727). */
+void fftshift_in_place(CF_VEC_1D& signal);
+void fftshift_in_place(CF_VEC_2D& signals);
+
+/* ECCM RFI detection and notch (signal_processing.cpp:913-1004); rx_pol
+ * is unused, as in the reference. */
+void eccm(CF_VEC_2D& signals, const int& fft_size, const int& stride, const double& detection_threshold,
+          const double& mitigation_threshold, const char& rx_pol = 'V');

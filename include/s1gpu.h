@@ -237,6 +237,47 @@ int s1g_focus_burst(s1g_ctx* ctx, const s1g_burst_desc* desc, int prec);
 #define S1G_BATCH_GRAPH 1
 int s1g_focus_batch(s1g_ctx* ctx, const s1g_burst_desc* descs, int n_bursts, int prec, int flags);
 
+/* ------------------------------------------- general signal processing */
+/* The reference's building blocks, for callers that use them one by one.
+ * All in the context stream, device pointers, precision per call. */
+
+/* DFT of every row (axis = 1, length n_cols) or every column (axis = 0,
+ * length n_rows) of an [n_rows][ld] array, in place: forward unnormalised
+ * e^{-2 pi i jk/n}, inverse e^{+...} times 1/n -- compute_1d_dft,
+ * compute_1d_dft_in_place, _compute_axis_dft, compute_axis_dft(_in_place)
+ * (signal_processing.cpp:516-681, FFTW conventions).  Any length: Stockham
+ * passes when n factors over radices up to 71, else Bluestein's chirp-z
+ * transform through a power-of-two length. */
+int s1g_dft(s1g_ctx* ctx, int prec, int n_rows, int n_cols, void* data, int64_t ld, int axis, int inverse);
+
+/* dst[c][r] = src[r][c] (transpose, signal_processing.cpp:463-498); out of place. */
+int s1g_transpose(s1g_ctx* ctx, int prec, int n_rows, int n_cols, const void* src, int64_t ld_src, void* dst,
+                  int64_t ld_dst);
+
+/* data[r][k] *= vec[k] (or conj(vec[k])) for k < n: the product of
+ * pulse_compression (image_formation.cpp:28-43) applied to every row. */
+int s1g_multiply(s1g_ctx* ctx, int prec, int n_rows, int n, void* data, int64_t ld, const void* vec, int conj_vec);
+
+/* out = conj(DFT(chirp)) / E, E = sum |c|^4 / n (get_reference_function,
+ * image_formation.cpp:4-25); out may equal chirp. */
+int s1g_reference_function(s1g_ctx* ctx, int prec, const void* chirp, int n, void* out);
+
+/* The replica chirp of L0Packet::get_replica_chirp (packet.cpp:345-373):
+ * n_samples + 2 nc values, nc = floor(fs txpl), zero except the nc samples
+ * (1/nc) exp(i 2 PI (txpsf t + txprr/2 t^2)) centred in the vector. */
+int s1g_replica_chirp(s1g_ctx* ctx, int prec, int n_samples, double txpsf_mhz, double txprr_mhz_us, double txpl_us,
+                      double fs_mhz, void* out);
+
+/* out[q] = sinc_interpolate(signal, t[q], half_width) for q < m
+ * (signal_processing.cpp:228-257): Hann-windowed sinc over the 2 L + 1
+ * samples around floor(t).  t: device doubles. */
+int s1g_sinc_interpolate(s1g_ctx* ctx, int prec, const void* signal, int len, const double* t, int m, int half_width,
+                         void* out);
+
+/* Host function: s1g_packet_info from decoded header field values (for
+ * callers holding the reference's header maps, packet.h:142-167). */
+int s1g_packet_info_from_fields(const int64_t* primary, const int64_t* secondary, s1g_packet_info* info);
+
 /* ----------------------------------------------------------- autofocus */
 /* (No reference counterpart.)  rd: range-Doppler data after RCMC and
  * matched filter, [n_lines][ld] (rows = Doppler bins).  phase: device

@@ -1,5 +1,5 @@
-"""Build recipe for the gfx950 library (libs1gpu.so) and the synthetic-scene
-library (synth/libs1synth.so).  Run ``python -m sentinel1_decode_amd.build``."""
+"""Build recipe for the gfx950 library (libs1gpu.so), the C++ drop-in surface
+over it (libs1dropin.so), and the synthetic-scene library (synth/libs1synth.so).  Run ``python -m sentinel1_decode_amd.build``."""
 from __future__ import annotations
 
 import glob
@@ -62,6 +62,35 @@ def build_gpu(verbose=False, jobs=8, force=True):
     return LIB
 
 
+DROPIN = os.path.join(HERE, "libs1dropin.so")
+CPP_TEST = os.path.join(HERE, "..", "tests", "cpp", "test_dropin")
+
+
+def build_dropin(verbose=False):
+    """The C++ drop-in surface (include/s1/*.h) over libs1gpu.so, and the
+    C++ test program that exercises it."""
+    inc = os.path.join(HERE, "..", "include")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    # host-only C++: g++ with the HIP runtime headers and library
+    host = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", inc]
+    tmp = DROPIN + ".tmp"
+    cmd = [*host, "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(rocm, "include"), "-shared", "-o", tmp,
+           os.path.join(HERE, "cpp", "dropin.cpp"), "-L", HERE, "-ls1gpu", "-L", os.path.join(rocm, "lib"),
+           "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DROPIN)
+    src = os.path.join(HERE, "..", "tests", "cpp", "test_dropin.cpp")
+    if os.path.exists(src):
+        cmd = [*host, "-o", CPP_TEST, src, "-L", HERE, "-ls1dropin", "-ls1gpu",
+               "-Wl,-rpath,$ORIGIN/../../sentinel1_decode_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return DROPIN
+
+
 def build_synth():
     from .synth import build as b
     return b(force=True)
@@ -69,6 +98,7 @@ def build_synth():
 
 def main():
     build_gpu(verbose="-v" in sys.argv, force="--force" in sys.argv)
+    build_dropin(verbose="-v" in sys.argv)
     build_synth()
     print(LIB)
 

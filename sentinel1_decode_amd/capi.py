@@ -84,6 +84,16 @@ _SIGS = [
                                  C.POINTER(C.c_double), C.c_void_p]),
     ("s1g_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
     ("s1g_profile_read", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("s1g_dft", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_int]),
+    ("s1g_transpose", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                C.c_int64]),
+    ("s1g_multiply", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int]),
+    ("s1g_reference_function", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    ("s1g_replica_chirp", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_void_p]),
+    ("s1g_sinc_interpolate", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                       C.c_void_p]),
+    ("s1g_packet_info_from_fields", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("s1g_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
     ("s1g_force_variant", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("s1g_last_variant", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -181,6 +191,56 @@ class Context:
         if rc:
             msg = lib().s1g_last_error(self.h)
             raise S1GError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ------------------------------------------------ general signal processing
+    def dft(self, x, axis=1, inverse=False):
+        """In-place DFT of a 2-D complex tensor along `axis` (FFTW conventions,
+        inverse scaled by 1/n)."""
+        self._bind_stream()
+        ld = x.stride(0) if x.shape[0] > 1 else x.shape[1]
+        self._check(lib().s1g_dft(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), ld, axis, int(inverse)),
+                    "dft")
+        return x
+
+    def transpose(self, x):
+        import torch
+        out = torch.empty((x.shape[1], x.shape[0]), dtype=x.dtype, device=x.device)
+        self._bind_stream()
+        self._check(lib().s1g_transpose(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), x.stride(0), _ptr(out),
+                                        out.stride(0)), "transpose")
+        return out
+
+    def multiply(self, x, vec, conj=False):
+        self._bind_stream()
+        self._check(lib().s1g_multiply(self.h, _prec_of(x), x.shape[0], vec.shape[0], _ptr(x), x.stride(0),
+                                       _ptr(vec), int(conj)), "multiply")
+        return x
+
+    def reference_function(self, chirp):
+        import torch
+        out = torch.empty_like(chirp)
+        self._bind_stream()
+        self._check(lib().s1g_reference_function(self.h, _prec_of(chirp), _ptr(chirp), chirp.shape[0], _ptr(out)),
+                    "reference_function")
+        return out
+
+    def replica_chirp(self, n_samples, txpsf, txprr, txpl, fs, dtype=None, device="cuda"):
+        import torch
+        dtype = dtype or torch.complex128
+        nc = int(np.floor(fs * txpl))
+        out = torch.empty(n_samples + 2 * nc, dtype=dtype, device=device)
+        self._bind_stream()
+        self._check(lib().s1g_replica_chirp(self.h, _prec_of(out), n_samples, txpsf, txprr, txpl, fs, _ptr(out)),
+                    "replica_chirp")
+        return out
+
+    def sinc_interpolate(self, signal, t, half_width=8):
+        import torch
+        out = torch.empty(t.shape[0], dtype=signal.dtype, device=signal.device)
+        self._bind_stream()
+        self._check(lib().s1g_sinc_interpolate(self.h, _prec_of(signal), _ptr(signal), signal.shape[0], _ptr(t),
+                                               t.shape[0], half_width, _ptr(out)), "sinc_interpolate")
+        return out
 
     # ------------------------------------------------------------- autotuner
     def set_autotune(self, on=True):

@@ -55,7 +55,7 @@ struct DifSeq<T, INV, R, Rest...> {
 template <typename T, bool INV, int NT, int W, int MAXE, int... Rs>
 __global__ __launch_bounds__(NT) void az_fft_fixed_kernel(cpx<T>* __restrict__ data, int64_t ld, int N,
                                                           int npanels, const cpx<T>* __restrict__ tw_g,
-                                                          const int32_t* __restrict__ perm_g, T scale) {
+                                                          const int32_t* __restrict__ perm_g, T scale, int sh) {
     constexpr int L = (Rs * ...);
     constexpr int total = L * W;
     static_assert(total <= NT * MAXE, "panel fits the per-thread registers");
@@ -87,8 +87,7 @@ __global__ __launch_bounds__(NT) void az_fft_fixed_kernel(cpx<T>* __restrict__ d
     }
     __syncthreads();
     DifSeq<T, INV, Rs...>::run(buf, L, 1, W, 1, W, tw, tid, NT);
-    // forward: out[i] = X[(i + L - L/2) mod L] (fftshift); inverse: x[k] / L
-    constexpr int sh = INV ? 0 : L - L / 2;
+    // forward with sh = L - L/2: out[i] = X[(i + sh) mod L] (fftshift); inverse: x[k] / L
 #pragma unroll
     for (int i = 0; i < MAXE; ++i) {
         const int idx = tid + NT * i;
@@ -103,7 +102,7 @@ __global__ __launch_bounds__(NT) void az_fft_fixed_kernel(cpx<T>* __restrict__ d
 }
 
 template <typename T, bool INV, int W, int... Rs>
-static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale) {
+static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
     constexpr int NT = 512;
     constexpr int L = (Rs * ...);
     constexpr int MAXE = (L * W + NT - 1) / NT;
@@ -111,7 +110,7 @@ static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64
     const int per = (npanels + 7) / 8;
     const size_t lds = (size_t)L * W * sizeof(cpx<T>) + (size_t)L * (sizeof(cpx<T>) + sizeof(int32_t));
     hipLaunchKernelGGL((az_fft_fixed_kernel<T, INV, NT, W, MAXE, Rs...>), dim3((unsigned)(8 * per)), dim3(NT), lds,
-                       s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale);
+                       s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale, sh);
 }
 
 // Whether a plan's radix sequence is one compiled into az_fft_fixed_kernel.
@@ -131,7 +130,7 @@ static int fixed_sequence(const FftPlan& p) {
 
 // Plans with a compiled radix sequence (the IW1/IW2/IW3 burst lengths).
 template <typename T, bool INV, int W>
-static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale) {
+static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
     const FftPlan& p = fl.plan;
     auto is = [&](std::initializer_list<int> rs) {
         if ((int)rs.size() != p.npass) return false;
@@ -140,9 +139,9 @@ static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, 
             if (p.radix[k++] != r) return false;
         return true;
     };
-    if (is({37, 5, 8})) launch_az_fixed<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale);
-    else if (is({71, 5, 4})) launch_az_fixed<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale);
-    else if (is({19, 5, 8, 2})) launch_az_fixed<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale);
+    if (is({37, 5, 8})) launch_az_fixed<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale, sh);
+    else if (is({71, 5, 4})) launch_az_fixed<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale, sh);
+    else if (is({19, 5, 8, 2})) launch_az_fixed<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale, sh);
     else return false;
     return true;
 }
@@ -150,7 +149,7 @@ static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, 
 template <typename T, bool INV, int NT>
 __global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__ data, int64_t ld, int L, int N, int W,
                                                     int npanels, FftPlan plan, const cpx<T>* __restrict__ tw,
-                                                    const int32_t* __restrict__ perm, T scale) {
+                                                    const int32_t* __restrict__ perm, T scale, int sh) {
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
     const int panel = xcd_panel(blockIdx.x, npanels);
@@ -168,7 +167,7 @@ __global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__
         }
         __syncthreads();
         fft_dit_lds<T, false>(buf, plan, W, 1, W, tw, tid, NT);
-        const int sh = L - L / 2;  // fftshift: out[i] = X[(i + L - L/2) mod L]
+        // sh = L - L/2 for the fftshift: out[i] = X[(i + L - L/2) mod L]
         for (int idx = tid; idx < total; idx += NT) {
             const int i = idx / W, v = idx - i * W;
             int src = i + sh;
@@ -192,7 +191,7 @@ __global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__
 }
 
 static int launch_az_generic(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
-                             int inverse) {
+                             int inverse, int sh) {
     constexpr int NT = 512;
     // panel width: 64-byte row segments (8 complex64 / 4 complex128) within the LDS budget
     int W = prec == S1G_F32 ? 8 : 4;
@@ -207,18 +206,18 @@ static int launch_az_generic(hipStream_t s, int prec, const DevFft& fl, void* da
         const float sc = (float)(1.0 / (double)L);
         if (inverse)
             hipLaunchKernelGGL((az_fft_generic_kernel<float, true, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L, N,
-                               W, npanels, fl.plan, fl.tw32, fl.perm, sc);
+                               W, npanels, fl.plan, fl.tw32, fl.perm, sc, sh);
         else
             hipLaunchKernelGGL((az_fft_generic_kernel<float, false, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L,
-                               N, W, npanels, fl.plan, fl.tw32, fl.perm, 1.0f);
+                               N, W, npanels, fl.plan, fl.tw32, fl.perm, 1.0f, sh);
     } else {
         const double sc = 1.0 / (L);
         if (inverse)
             hipLaunchKernelGGL((az_fft_generic_kernel<double, true, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
-                               N, W, npanels, fl.plan, fl.tw64, fl.perm, sc);
+                               N, W, npanels, fl.plan, fl.tw64, fl.perm, sc, sh);
         else
             hipLaunchKernelGGL((az_fft_generic_kernel<double, false, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
-                               N, W, npanels, fl.plan, fl.tw64, fl.perm, 1.0);
+                               N, W, npanels, fl.plan, fl.tw64, fl.perm, 1.0, sh);
     }
     return (int)hipGetLastError();
 }
@@ -229,23 +228,24 @@ int azimuth_fft_variants(const DevFft& fl) { return fixed_sequence(fl.plan) ? 3 
 // panels (twice the workgroups per CU), 2 = the generic kernel; otherwise
 // only the generic kernel.
 int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
-                       int inverse, int variant) {
+                       int inverse, int variant, int shift) {
+    const int sh = (!inverse && shift) ? L - L / 2 : 0;
     if (fixed_sequence(fl.plan) && variant < 2) {
         const bool wide = variant == 0;
         if (prec == S1G_F32) {
             const float sc = (float)(1.0 / (double)L);
             cpx<float>* d = (cpx<float>*)data;
-            if (wide) inverse ? az_fixed<float, true, 8>(s, fl, d, ld, N, sc) : az_fixed<float, false, 8>(s, fl, d, ld, N, 1.0f);
-            else inverse ? az_fixed<float, true, 4>(s, fl, d, ld, N, sc) : az_fixed<float, false, 4>(s, fl, d, ld, N, 1.0f);
+            if (wide) inverse ? az_fixed<float, true, 8>(s, fl, d, ld, N, sc, sh) : az_fixed<float, false, 8>(s, fl, d, ld, N, 1.0f, sh);
+            else inverse ? az_fixed<float, true, 4>(s, fl, d, ld, N, sc, sh) : az_fixed<float, false, 4>(s, fl, d, ld, N, 1.0f, sh);
         } else {
             const double sc = 1.0 / (double)L;
             cpx<double>* d = (cpx<double>*)data;
-            if (wide) inverse ? az_fixed<double, true, 4>(s, fl, d, ld, N, sc) : az_fixed<double, false, 4>(s, fl, d, ld, N, 1.0);
-            else inverse ? az_fixed<double, true, 2>(s, fl, d, ld, N, sc) : az_fixed<double, false, 2>(s, fl, d, ld, N, 1.0);
+            if (wide) inverse ? az_fixed<double, true, 4>(s, fl, d, ld, N, sc, sh) : az_fixed<double, false, 4>(s, fl, d, ld, N, 1.0, sh);
+            else inverse ? az_fixed<double, true, 2>(s, fl, d, ld, N, sc, sh) : az_fixed<double, false, 2>(s, fl, d, ld, N, 1.0, sh);
         }
         return (int)hipGetLastError();
     }
-    return launch_az_generic(s, prec, fl, data, ld, L, N, inverse);
+    return launch_az_generic(s, prec, fl, data, ld, L, N, inverse, sh);
 }
 
 // ----------------------------------------------------------- column table

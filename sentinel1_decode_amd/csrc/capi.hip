@@ -67,6 +67,13 @@ struct s1g_ctx {
     std::vector<Mark> marks;
     double stage_ms[S1G_NSTAGES] = {0};
     int64_t stage_n[S1G_NSTAGES] = {0};
+    // general DSP entry points
+    void* dft_ws = nullptr;
+    size_t dft_ws_bytes = 0;
+    void* blue_scratch = nullptr;
+    size_t blue_scratch_bytes = 0;
+    std::map<std::tuple<int, int, int>, void*> blue_filters;  // (n, prec, inverse)
+    double* dsp_scal = nullptr;
     // autotuner
     bool autotune = true;
     int force[S1G_NSTAGES] = {-1, -1, -1, -1, -1, -1, -1, -1};
@@ -482,6 +489,10 @@ int s1g_destroy(s1g_ctx* c) {
     hipFree(c->eccm_scratch);
     hipFree(c->af_scalars);
     hipFree(c->af_img);
+    hipFree(c->dft_ws);
+    hipFree(c->blue_scratch);
+    for (auto& kv : c->blue_filters) hipFree(kv.second);
+    hipFree(c->dsp_scal);
     if (c->own) hipStreamDestroy(c->own);
     delete c;
     return S1G_OK;
@@ -581,6 +592,8 @@ static uint64_t be_field(const uint8_t* p, int64_t bit, int n) {
     return v;
 }
 
+static int derive_packet_info(s1g_packet_info* info);
+
 int s1g_parse_packet_header(const uint8_t* pk, int64_t nbytes, s1g_packet_info* info) {
     if (!pk || !info || nbytes < 68) return S1G_EARG;
     memset(info, 0, sizeof *info);
@@ -594,6 +607,19 @@ int s1g_parse_packet_header(const uint8_t* pk, int64_t nbytes, s1g_packet_info* 
         info->secondary[i] = (int64_t)be_field(pk, bit, kSecW[i]);
         bit += kSecW[i];
     }
+    return derive_packet_info(info);
+}
+
+int s1g_packet_info_from_fields(const int64_t* primary, const int64_t* secondary, s1g_packet_info* info) {
+    if (!primary || !secondary || !info) return S1G_EARG;
+    memset(info, 0, sizeof *info);
+    memcpy(info->primary, primary, sizeof info->primary);
+    memcpy(info->secondary, secondary, sizeof info->secondary);
+    return derive_packet_info(info);
+}
+
+// Quantities derived from the header fields (packet.cpp:29-100, 265-281).
+static int derive_packet_info(s1g_packet_info* info) {
     const int64_t* s = info->secondary;
     info->num_quads = (int32_t)s[44];
     info->baq_mode = (int32_t)s[15];
@@ -711,6 +737,137 @@ int s1g_eccm(s1g_ctx* c, const s1g_eccm_desc* d, int prec, void* data, int64_t l
     HIPCHK(c, hipMemcpy2DAsync(data, esize(prec) * ld, c->ws, row, row, d->n_lines, hipMemcpyDeviceToDevice,
                                c->stream));
     return S1G_OK;
+}
+
+// ------------------------------------------------------------ general DSP
+// Rows x n DFT along rows, in place: Stockham passes when n factors over
+// the pass radices, else Bluestein through a power-of-two length.
+static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, int inverse) {
+    if (rows <= 0 || n <= 0) return 0;
+    const DevFft* f;
+    int rc = get_fft(c, n, &f);
+    if (rc) return rc;
+    const size_t es = esize(prec);
+    const double scale = inverse ? 1.0 / (double)n : 1.0;
+    if (f->plan.npass >= 0) {
+        rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 2 * es * (size_t)rows * ld);
+        if (rc) return rc;
+        char* w1 = (char*)c->dft_ws;
+        char* w2 = w1 + es * (size_t)rows * ld;
+        rc = launch_fft_global(c->stream, prec, *f, data, w1, w2, rows, ld, inverse);
+        if (!rc) rc = launch_copy_scale(c->stream, prec, w1, ld, data, ld, rows, n, scale);
+        return check_launch(c, rc, "dft");
+    }
+    int M = 1;
+    while (M < 2 * n - 1) M <<= 1;
+    const DevFft* fM;
+    rc = get_plan(c, M, &fM);
+    if (rc) return rc;
+    auto key = std::make_tuple(n, prec, inverse ? 1 : 0);
+    auto it = c->blue_filters.find(key);
+    void* bhat;
+    if (it != c->blue_filters.end()) {
+        bhat = it->second;
+    } else {
+        rc = ensure(c, &c->blue_scratch, &c->blue_scratch_bytes, sizeof(cpx<double>) * 3 * (size_t)M);
+        if (rc) return rc;
+        HIPCHK(c, hipMalloc(&bhat, es * M));
+        rc = launch_bluestein_filter(c->stream, prec, *fM, n, inverse, c->blue_scratch, bhat);
+        if (rc) {
+            hipFree(bhat);
+            return check_launch(c, rc, "dft filter");
+        }
+        c->blue_filters[key] = bhat;
+    }
+    rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 3 * es * (size_t)rows * M);
+    if (rc) return rc;
+    char* w1 = (char*)c->dft_ws;
+    char* w2 = w1 + es * (size_t)rows * M;
+    char* w3 = w2 + es * (size_t)rows * M;
+    return check_launch(c, launch_bluestein(c->stream, prec, *fM, bhat, data, ld, rows, n, inverse, scale, w1, w2, w3),
+                        "dft");
+}
+
+// Along columns: the LDS column kernel when the length has a plan, else a
+// transpose to rows and back.
+static int dft_cols(s1g_ctx* c, int prec, int rows, int cols, void* data, int64_t ld, int inverse) {
+    if (rows <= 0 || cols <= 0) return 0;
+    const DevFft* fl;
+    int rc = get_fft(c, rows, &fl);
+    if (rc) return rc;
+    if (fl->plan.npass >= 0) {
+        rc = launch_azimuth_fft(c->stream, prec, *fl, data, ld, rows, cols, inverse, 0, 0);
+        if (rc != -3) return check_launch(c, rc, "dft");
+    }
+    void* t = nullptr;
+    const size_t es = esize(prec);
+    HIPCHK(c, hipMalloc(&t, es * (size_t)rows * cols));
+    rc = launch_transpose(c->stream, prec, data, ld, t, rows, rows, cols);
+    if (!rc) rc = dft_rows(c, prec, cols, rows, t, rows, inverse);
+    if (!rc) rc = launch_transpose(c->stream, prec, t, rows, data, ld, cols, rows);
+    (void)hipStreamSynchronize(c->stream);
+    hipFree(t);
+    return rc > 0 ? check_launch(c, rc, "dft") : rc;
+}
+
+extern "C" int s1g_dft(s1g_ctx* c, int prec, int n_rows, int n_cols, void* data, int64_t ld, int axis, int inverse) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (n_rows < 0 || n_cols < 0 || (axis != 0 && axis != 1)) return fail(c, S1G_EARG, "bad DFT shape or axis%s");
+    if (!data || ld < n_cols) return fail(c, S1G_EARG, "bad DFT buffer%s");
+    return axis == 1 ? dft_rows(c, prec, n_rows, n_cols, data, ld, inverse != 0)
+                     : dft_cols(c, prec, n_rows, n_cols, data, ld, inverse != 0);
+}
+
+extern "C" int s1g_transpose(s1g_ctx* c, int prec, int n_rows, int n_cols, const void* src, int64_t ld_src, void* dst,
+                             int64_t ld_dst) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (n_rows < 0 || n_cols < 0 || !src || !dst || src == dst || ld_src < n_cols || ld_dst < n_rows)
+        return fail(c, S1G_EARG, "bad transpose arguments%s");
+    return check_launch(c, launch_transpose(c->stream, prec, src, ld_src, dst, ld_dst, n_rows, n_cols), "transpose");
+}
+
+extern "C" int s1g_multiply(s1g_ctx* c, int prec, int n_rows, int n, void* data, int64_t ld, const void* vec,
+                            int conj_vec) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (n_rows < 0 || n < 0 || !data || !vec || ld < n) return fail(c, S1G_EARG, "bad multiply arguments%s");
+    return check_launch(c, launch_mul_rows(c->stream, prec, data, ld, vec, n_rows, n, conj_vec != 0), "multiply");
+}
+
+extern "C" int s1g_reference_function(s1g_ctx* c, int prec, const void* chirp, int n, void* out) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (n <= 0 || !chirp || !out) return fail(c, S1G_EARG, "bad reference-function arguments%s");
+    if (!c->dsp_scal) HIPCHK(c, hipMalloc((void**)&c->dsp_scal, 4 * sizeof(double)));
+    int rc = launch_chirp_energy(c->stream, prec, chirp, n, c->dsp_scal);
+    if (rc) return check_launch(c, rc, "reference function");
+    if (out != chirp) HIPCHK(c, hipMemcpyAsync(out, chirp, esize(prec) * n, hipMemcpyDeviceToDevice, c->stream));
+    rc = dft_rows(c, prec, 1, n, out, n, 0);
+    if (rc) return rc;
+    return check_launch(c, launch_conj_div(c->stream, prec, out, n, c->dsp_scal), "reference function");
+}
+
+extern "C" int s1g_replica_chirp(s1g_ctx* c, int prec, int n_samples, double txpsf_mhz, double txprr_mhz_us,
+                                 double txpl_us, double fs_mhz, void* out) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    const int nc = (int)floor(fs_mhz * txpl_us);
+    if (n_samples < 0 || nc < 2 || !out) return fail(c, S1G_EARG, "bad replica-chirp arguments%s");
+    return check_launch(c, launch_replica_chirp(c->stream, prec, out, n_samples + 2 * nc, nc, txpsf_mhz, txprr_mhz_us,
+                                                txpl_us),
+                        "replica chirp");
+}
+
+extern "C" int s1g_sinc_interpolate(s1g_ctx* c, int prec, const void* signal, int len, const double* t, int m,
+                                    int half_width, void* out) {
+    if (!c) return S1G_EARG;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (len < 0 || m < 0 || half_width < 1 || (m && (!signal || !t || !out)))
+        return fail(c, S1G_EARG, "bad sinc-interpolation arguments%s");
+    return check_launch(c, launch_sinc_interp(c->stream, prec, signal, len, t, m, half_width, out),
+                        "sinc interpolation");
 }
 
 static int burst_chain(s1g_ctx* c, const s1g_burst_desc& b, int prec) {

@@ -61,8 +61,10 @@ int launch_range_os(hipStream_t stream, int prec, const DevFft& fm, const void* 
 // launch with a given variant.
 int azimuth_fft_variants(const DevFft& fl);
 constexpr int kRcmcVariants = 3;
+// shift = 1: fftshift after the forward transform (the azimuth chain);
+// shift = 0: plain transform.  The inverse is scaled by 1/n_lines.
 int launch_azimuth_fft(hipStream_t stream, int prec, const DevFft& fl, void* data, int64_t ld, int n_lines,
-                       int n_samples, int inverse, int variant);
+                       int n_samples, int inverse, int variant, int shift = 1);
 int launch_az_columns(hipStream_t stream, const s1g_azimuth_desc& d, AzColumn* cols);
 int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
                    int64_t ld_in, void* out, int64_t ld_out, int variant);
@@ -74,6 +76,25 @@ int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, cons
 int launch_eccm(hipStream_t stream, int prec, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,
                 const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
                 void* partial);
+
+// General signal processing (dsp.hip).
+int launch_copy_scale(hipStream_t stream, int prec, const void* src, int64_t lds, void* dst, int64_t ldd, int rows,
+                      int n, double scale);
+int launch_transpose(hipStream_t stream, int prec, const void* src, int64_t lds, void* dst, int64_t ldd, int rows,
+                     int cols);
+// Bluestein: filter spectrum for an n-point transform through M points
+// (scratch 3 M complex doubles), then the transform of `rows` rows of x in
+// place (w1..w3: rows * M complex each); `scale` multiplies the result.
+int launch_bluestein_filter(hipStream_t stream, int prec, const DevFft& fM, int n, int inverse, void* scratch,
+                            void* bhat);
+int launch_bluestein(hipStream_t stream, int prec, const DevFft& fM, const void* bhat, void* x, int64_t ldx, int rows,
+                     int n, int inverse, double scale, void* w1, void* w2, void* w3);
+int launch_mul_rows(hipStream_t stream, int prec, void* a, int64_t lda, const void* v, int rows, int n, int conj_v);
+int launch_chirp_energy(hipStream_t stream, int prec, const void* chirp, int n, double* energy);
+int launch_conj_div(hipStream_t stream, int prec, void* ref, int n, const double* energy);
+int launch_replica_chirp(hipStream_t stream, int prec, void* out, int len, int nc, double txpsf, double txprr,
+                         double txpl);
+int launch_sinc_interp(hipStream_t stream, int prec, const void* x, int len, const double* t, int m, int L, void* out);
 
 // Autofocus.  work: af_work_doubles() doubles; scal: 3 doubles (S, T, E);
 // img: n_lines * n_samples complex (dense).

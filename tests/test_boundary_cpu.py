@@ -103,3 +103,44 @@ def test_create_without_gpu_reports_error():
     h = C.c_void_p()
     rc = capi.lib().s1g_create(C.byref(h), 0)
     assert rc == -2 and not h.value
+
+
+def test_cpp_dropin_headers_compile_standalone(tmp_path):
+    """include/s1/*.h are plain C++17 (no HIP headers): a reference caller
+    compiles against them with -I include/s1."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        '#include "packet.h"\n#include "signal_processing.h"\n#include "image_formation.h"\n'
+        '#include "s1_decoder.h"\n'
+        "int main() { CF_VEC_1D v(8); PACKET_VEC_1D p; (void)v; (void)p;\n"
+        "  CF_VEC_1D (*f)(const CF_VEC_1D&, int, const bool&) = compute_1d_dft; (void)f; return 0; }\n")
+    import subprocess
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include", "s1"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_cpp_dropin_library_exports_reference_names():
+    import subprocess
+    lib = os.path.join(ROOT, "sentinel1_decode_amd", "libs1dropin.so")
+    assert os.path.exists(lib), "build the drop-in: python -m sentinel1_decode_amd.build"
+    syms = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True).stdout
+    for name in ("decode_packets_in_place(", "decode_packets(", "L0Packet::get_signal()",
+                 "L0Packet::get_replica_chirp() const", "compute_1d_dft(", "compute_1d_dft_in_place(",
+                 "_compute_axis_dft(", "compute_axis_dft(", "compute_axis_dft_in_place(", "transpose(",
+                 "fftshift_in_place(", "sinc_interpolate(", "eccm(", "pulse_compression(",
+                 "get_reference_function(", "linspace(", "range_compress_burst(", "azimuth_compress_burst("):
+        assert name in syms, name
+
+
+def test_cpp_dropin_fails_loudly_without_gpu():
+    """No CPU fallback: the drop-in raises when no GPU is present."""
+    import subprocess
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    exe = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "FAIL exception" in r.stdout
