@@ -10,6 +10,7 @@ oracle (CPU FP64 restatement) is timed on the host cores as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
     python bench.py --copy-calibration   # plain copy for HBM counter scaling
+    python bench.py --workload {iw-batch,iw2-swath-eccm,autofocus}   # the other workloads
 """
 from __future__ import annotations
 
@@ -135,6 +136,184 @@ def run_cpu_baseline(burst):
     return out, t1 - t0
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def run_iw_batch(steps, warmup):
+    """27 uneven bursts (9 IW1-like 1420 x 21600, 9 IW2-like 1480 x 24000,
+    9 IW3-like 1520 x 20400; 200 Rayleigh targets each, clutter 0.1) through
+    the FP32 chain: one captured hipGraph for the whole batch against
+    launching the bursts one after another."""
+    import torch
+    from sentinel1_decode_amd import capi, pipeline, synth
+    t0 = time.perf_counter()
+    dbs = []
+    for sp in synth.config3_specs():
+        dbs.append(pipeline.from_synth(synth.make_burst(sp)))
+    print(f"# 27 bursts synthesised and uploaded in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    ctx = capi.Context(0)
+    outs = [torch.empty((db.n_lines, db.n_samples), dtype=torch.complex64, device="cuda") for db in dbs]
+    descs = [db.burst_desc(o) for db, o in zip(dbs, outs)]
+    samples = sum(db.n_lines * db.n_samples for db in dbs)
+    ms_graph = _timed(lambda: ctx.focus_batch(descs, capi.S1G_F32, graph=True), steps, warmup)
+    ms_seq = _timed(lambda: [db.focus(ctx, capi.S1G_F32, out=o) for db, o in zip(dbs, outs)], steps, warmup)
+    failed = sum(int((db.d_status.cpu().numpy() != 0).sum()) for db in dbs)
+    ctx.close()
+    return {"metric": "focused Msamples/s, 27 uneven IW bursts in one hipGraph (FP32)",
+            "value": round(samples / (ms_graph * 1e-3) / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": round(ms_graph, 4), "higher_is_better": True,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "27 bursts: 9x 1420x21600 (fs 50.05 MHz, PRF 1685), 9x 1480x24000 "
+                                   "(64.35 MHz, 1717), 9x 1520x20400 (54.60 MHz, 1451)", "samples": samples},
+            "sequential_ms_per_step": round(ms_seq, 4),
+            "sequential_msamples_s": round(samples / (ms_seq * 1e-3) / 1e6, 3),
+            "graph_speedup": round(ms_seq / ms_graph, 3), "packets_failed": failed}
+
+
+def run_swath_eccm(steps, warmup, n_bursts=9):
+    """An IW2 swath of 9 bursts (1480 x 24000 each) with narrowband RFI on
+    20 % of the lines (runs of 10-50 lines, 3 tones within +-28 MHz, INR
+    15-25 dB); ECCM thresholds from clean-line statistics; FP32 chain with
+    ECCM, the 9 bursts in one captured graph."""
+    import torch
+    from sentinel1_decode_amd import capi, pipeline, synth
+    ctx = capi.Context(0)
+    dbs, masks = [], []
+    t0 = time.perf_counter()
+    for k in range(n_bursts):
+        sp = synth.config1(seed=2 * 1000 + k)
+        raw = sp.raw()
+        masks.append(synth.add_rfi(raw, sp, seed=2 * 1000 + k))
+        dbs.append(pipeline.from_synth(synth.encode_burst(sp, raw)))
+        del raw
+    print(f"# swath synthesised in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    # thresholds from the lines the RFI generator left clean (first burst)
+    dec, _ = dbs[0].decode(ctx, capi.S1G_F32)
+    clean = torch.from_numpy(~masks[0]).cuda()
+    spec_max = torch.fft.fft(dec, dim=1).abs().amax(dim=1)
+    det = 1.25 * float(spec_max[clean].max())
+    win = dec[clean][:64, : (dec.shape[1] // 64 - 1) * 64].unfold(1, 128, 64)
+    w = torch.fft.fft(win, dim=-1)
+    grad = (w[..., 2:] - w[..., :-2]).abs() / 2
+    mit = 2.0 * float(grad.max())
+    del dec, win, w, grad
+    eccm = capi.EccmDesc(dbs[0].n_lines, dbs[0].n_samples, 128, 64, det, mit)
+    outs = [torch.empty((db.n_lines, db.n_samples), dtype=torch.complex64, device="cuda") for db in dbs]
+    flags = [torch.zeros(db.n_lines, dtype=torch.uint8, device="cuda") for db in dbs]
+    descs = [db.burst_desc(o, eccm=eccm, eccm_flags=f) for db, o, f in zip(dbs, outs, flags)]
+    ms = _timed(lambda: ctx.focus_batch(descs, capi.S1G_F32, graph=True), steps, warmup)
+    fl = np.concatenate([f.cpu().numpy().astype(bool) for f in flags])
+    mk = np.concatenate(masks)
+    samples = sum(db.n_lines * db.n_samples for db in dbs)
+    ctx.close()
+    return {"metric": "focused Msamples/s, IW2 swath of 9 bursts with RFI, ECCM + FP32 chain",
+            "value": round(samples / (ms * 1e-3) / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"{n_bursts} x 1480x24000, RFI on 20% of lines", "samples": samples},
+            "eccm_thresholds": {"detection": det, "mitigation": mit},
+            "eccm_flag_recall": round(float((fl & mk).sum() / max(mk.sum(), 1)), 4),
+            "eccm_flag_precision": round(float((fl & mk).sum() / max(fl.sum(), 1)), 4)}
+
+
+def run_autofocus(iters=50):
+    """Differentiable autofocus on a 4096 x 8192 burst (200 targets, clutter
+    0.1): a quadratic phase error (2 rad at the aperture edge) plus a
+    Gaussian random walk (0.05 rad / Doppler bin) is injected into the
+    matched-filtered range-Doppler spectrum, and `iters` L-BFGS iterations
+    driven by the hand-written FP32 adjoint remove it.  The spectrum is the
+    checkpoint: each evaluation recomputes the image from it."""
+    import torch
+    from sentinel1_decode_amd import capi, pipeline, synth
+    ctx = capi.Context(0)
+    spec = synth.config4()
+    db = pipeline.from_synth(synth.make_burst(spec))
+    dec, _ = db.decode(ctx, capi.S1G_F32)
+    rc = ctx.range_compress(dec, db.geo.range_desc())
+    ctx.azimuth_fft(rc, inverse=False)
+    rd = ctx.rcmc_matched_filter(rc, db.azimuth_desc())
+    del dec, rc
+    L = rd.shape[0]
+    rng = np.random.default_rng(4)
+    f = np.linspace(-1.0, 1.0, L)
+    err = 2.0 * f ** 2 + np.cumsum(rng.normal(0.0, 0.05, L))
+    rd_err = (rd * torch.from_numpy(np.exp(1j * err)).to(rd.dtype).cuda()[:, None]).contiguous()
+    evals = [0]
+
+    def solve(data):
+        phase = torch.zeros(L, dtype=torch.float64, device="cuda", requires_grad=True)
+        opt = torch.optim.LBFGS([phase], lr=1.0, max_iter=iters, history_size=10, line_search_fn="strong_wolfe")
+
+        def closure():
+            opt.zero_grad()
+            g = torch.zeros(L, dtype=torch.float64, device="cuda")
+            e = ctx.af_adjoint(data, phase.detach(), g)
+            phase.grad = g
+            evals[0] += 1
+            return torch.tensor(e, dtype=torch.float64)
+
+        opt.step(closure)
+        return phase.detach()
+
+    zero = torch.zeros(L, dtype=torch.float64, device="cuda")
+    e0 = ctx.af_forward(rd_err, zero)
+    t0 = time.perf_counter()
+    est = solve(rd_err)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    n_eval = evals[0]
+    e1 = ctx.af_forward(rd_err, est)
+    est_clean = solve(rd).cpu().numpy()
+    est = est.cpu().numpy()
+    # gradient check at the solution: FP32 adjoint vs FP64 adjoint and vs
+    # central differences (eps 1e-3, 32 seeded bins) of the FP64 entropy
+    rd64 = rd_err.to(torch.complex128)
+    g32 = torch.zeros(L, dtype=torch.float64, device="cuda")
+    g64 = torch.zeros(L, dtype=torch.float64, device="cuda")
+    d_est = torch.from_numpy(est).cuda()
+    ctx.af_adjoint(rd_err, d_est, g32)
+    ctx.af_adjoint(rd64, d_est, g64)
+    g32, g64 = g32.cpu().numpy(), g64.cpu().numpy()
+    fd_err = 0.0
+    for i in np.random.default_rng(9).choice(L, 32, replace=False):
+        p1, p2 = est.copy(), est.copy()
+        p1[i] += 1e-3
+        p2[i] -= 1e-3
+        fd = (ctx.af_forward(rd64, torch.from_numpy(p1).cuda()) - ctx.af_forward(rd64, torch.from_numpy(p2).cuda())) / 2e-3
+        fd_err = max(fd_err, abs(fd - g64[i]) / max(np.abs(g64).max(), 1e-300))
+    ctx.close()
+
+    def rms(x, band=0.75):
+        i = np.arange(x.size, dtype=np.float64)
+        keep = np.abs(f) <= band
+        a = np.vstack([np.ones_like(i), i]).T[keep]
+        coef, *_ = np.linalg.lstsq(a, x[keep], rcond=None)
+        return float(np.sqrt(np.mean((x[keep] - a @ coef) ** 2)))
+
+    return {"metric": "autofocus: residual azimuth phase RMS after 50 L-BFGS iterations (4096 x 8192)",
+            "value": round(rms(err + est, 0.25), 5), "unit": "rad", "higher_is_better": False, "n_gpus": 1,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "4096x8192, 200 targets, clutter 0.1; error 2 rad quadratic + 0.05 rad/bin "
+                                   "random walk", "iterations": iters},
+            "residual_rms_rad": {"injected_band0.75": rms(err), "after_band0.75": rms(err + est),
+                                 "injected_band0.25": rms(err, 0.25), "after_band0.25": rms(err + est, 0.25),
+                                 "differential_vs_errorfree_autofocus_band0.75": rms(err + est - est_clean)},
+            "entropy": {"with_error": e0, "autofocused": e1},
+            "seconds": round(secs, 3), "entropy_gradient_evaluations": n_eval,
+            "ms_per_evaluation": round(secs * 1e3 / max(n_eval, 1), 3),
+            "gradient_check": {"rel_l2_fp32_vs_fp64_adjoint": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+                               "max_fd_error_rel_to_max_grad": fd_err}}
+
+
 def copy_calibration(mib=256, reps=5):
     """A plain device copy of known size (torch elementwise kernel: read
     mib MiB, write mib MiB per launch), the reference against which
@@ -160,9 +339,21 @@ def main():
     ap.add_argument("--samples", type=int, default=24000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain copy of known size (for counter calibration)")
+    ap.add_argument("--workload", default="iw2-burst",
+                    choices=["iw2-burst", "iw-batch", "iw2-swath-eccm", "autofocus"],
+                    help="iw2-burst (the default line); the others print their own line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload != "iw2-burst":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        fn = {"iw-batch": lambda: run_iw_batch(args.steps, args.warmup),
+              "iw2-swath-eccm": lambda: run_swath_eccm(args.steps, args.warmup),
+              "autofocus": lambda: run_autofocus()}[args.workload]
+        print(json.dumps(fn()), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); bursts shard across GPUs without "
