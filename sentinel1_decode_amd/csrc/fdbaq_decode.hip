@@ -8,7 +8,9 @@
 //
 // Work split inside the wavefront:
 //   stage   the packet's user data is copied into LDS as big-endian 32-bit
-//           words (all 64 lanes, coalesced);
+//           words (all 64 lanes, coalesced), up to 20 KB per packet so that
+//           occupancy does not depend on the launch's largest packet; words
+//           beyond that are converted from global memory when read;
 //   phase 1 the block structure is walked channel by channel (IE, IO, QE, QO):
 //           BRC/THIDX headers are read and the end of each block's symbols
 //           is found by a speculative, self-synchronising walk (walk_block):
@@ -23,6 +25,8 @@
 #include "s1g_device.h"
 #include "s1g_fdbaq.h"
 #include "s1g_internal.h"
+
+#include <algorithm>
 
 namespace s1g {
 
@@ -70,16 +74,53 @@ constexpr int kPadWords = 72;
 // kOverlap + 32 (the widest slice) + 11 (the longest symbol) <= 64 keeps
 // every walk inside one 64-bit window.
 constexpr int kOverlap = 16;
+constexpr int kOverlapB4 = 28;
 
 // Packets with more BAQ blocks than this are rejected (S1G_PKT_ERR_ARG):
 // the block BRCs are kept in 4 registers per lane.
 constexpr int kMaxBlocks = 256;
 
-// `n` (1..32) bits starting at bit `pos` of the big-endian word array.
-__device__ __forceinline__ uint32_t peek_bits(const uint32_t* w, int64_t pos, int n) {
+// Words of LDS a wavefront stages: packets up to this size (20 KB) are
+// staged whole, so at least 7 wavefronts fit a CU whatever the largest
+// packet of the launch; words beyond it come from global memory on demand.
+constexpr int kLdsWords = 5120;
+
+// A packet's user data as big-endian 32-bit words: word i holds bytes
+// 4i..4i+3, zero past the packet (and word `wlim` onwards).  Words below
+// `cap` are read from the LDS copy, the rest converted from global memory.
+template <bool ALL>
+struct PacketWords {
+    const uint32_t* lds;
+    int cap;
+    const uint32_t* al;  // 4-byte aligned, at or before the packet's first byte
+    int off;             // packet start - al, bytes
+    int64_t last_al;     // last aligned word touching the packet
+    int64_t nbytes;
+    int64_t wlim;
+
+    __device__ __forceinline__ uint32_t global_word(int64_t i) const {
+        if (4 * i >= nbytes) return 0u;
+        const uint32_t lo = (i <= last_al) ? al[i] : 0u;
+        const uint32_t hi = (off && i + 1 <= last_al) ? al[i + 1] : 0u;
+        const uint64_t v = ((uint64_t)hi << 32) | lo;
+        uint32_t be = __builtin_bswap32((uint32_t)(v >> (8 * off)));
+        const int64_t valid = nbytes - 4 * i;  // bytes of this word inside the packet
+        if (valid < 4) be &= ~(0xFFFFFFFFu >> (8 * valid));
+        return be;
+    }
+    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
+        if (i >= wlim) return 0u;
+        if constexpr (ALL) return lds[i];  // the whole packet (and padding) is staged
+        return i < cap ? lds[i] : global_word(i);
+    }
+};
+
+// `n` (1..32) bits starting at bit `pos`.
+template <bool ALL>
+__device__ __forceinline__ uint32_t peek_bits(const PacketWords<ALL>& w, int64_t pos, int n) {
     const int64_t wi = pos >> 5;
     const int sh = (int)(pos & 31);
-    uint64_t v = ((uint64_t)w[wi] << 32) | (uint64_t)w[wi + 1];
+    uint64_t v = ((uint64_t)w(wi) << 32) | (uint64_t)w(wi + 1);
     return (uint32_t)((v << sh) >> (64 - n));
 }
 
@@ -167,23 +208,26 @@ __device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
 // *hdr receives the 8 bits that follow the end (the next block's BRC or
 // THIDX field).  Reads stay below word `wlim`; once `start` passes `nbits`
 // the block cannot end inside the packet and the walk stops there.
-template <bool B4>
-__device__ int64_t walk_block(const uint32_t* __restrict__ words, int64_t wlim, int64_t nbits, int64_t start,
-                              uint64_t tab, int lo1, int hi1, int nsym, int delta, int lane, uint32_t* hdr) {
+template <bool B4, bool ALL>
+__device__ int64_t walk_block(const PacketWords<ALL>& words, int64_t nbits, int64_t start, uint64_t tab, int lo1,
+                              int hi1, int nsym, int delta, int lane, uint32_t* hdr) {
     for (;;) {
         if (start > nbits) {
             *hdr = 0;
             return start;
         }
         const int64_t s = start + (int64_t)lane * delta;
-        const int64_t org = max(s - kOverlap, start);
+        // BRC 4 codes resynchronise more slowly: a longer lead-in, within the
+        // 64-bit window (lead-in + delta + 11 <= 64)
+        const int lead = B4 ? min(kOverlapB4, 53 - delta) : kOverlap;
+        const int64_t org = max(s - lead, start);
         const int64_t w0 = org >> 5;
         Win5 w;
-        w.w0 = w0 < wlim ? words[w0] : 0u;
-        w.w1 = w0 + 1 < wlim ? words[w0 + 1] : 0u;
-        w.w2 = w0 + 2 < wlim ? words[w0 + 2] : 0u;
-        w.w3 = w0 + 3 < wlim ? words[w0 + 3] : 0u;
-        w.w4 = w0 + 4 < wlim ? words[w0 + 4] : 0u;
+        w.w0 = words(w0);
+        w.w1 = words(w0 + 1);
+        w.w2 = words(w0 + 2);
+        w.w3 = words(w0 + 3);
+        w.w4 = words(w0 + 4);
         // window offset of slice offset o is o + zero (zero <= 31 + kOverlap)
         const int zero = (int)(org & 31) + (int)(s - org);
         uint64_t win = win64(w, (int)(org & 31));
@@ -269,20 +313,21 @@ __device__ __forceinline__ int brc_of(int b, uint32_t v0, uint32_t v1, uint32_t 
     return __builtin_amdgcn_readlane((int)v, l);
 }
 
-template <typename T>
+template <typename T, bool ALL>
 __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
                                                           const uint64_t* __restrict__ offs,
                                                           const uint32_t* __restrict__ lens,
                                                           const int32_t* __restrict__ nqs, int n_packets,
-                                                          int max_words, int max_blocks, T* __restrict__ out,
-                                                          int64_t ld_out, int32_t* __restrict__ status) {
+                                                          int max_words, int lds_words, int max_blocks,
+                                                          T* __restrict__ out, int64_t ld_out,
+                                                          int32_t* __restrict__ status) {
     extern __shared__ uint32_t smem[];
     const int p = blockIdx.x;
     const int lane = threadIdx.x;
     if (p >= n_packets) return;
 
-    uint32_t* words = smem;                                      // max_words + kPadWords
-    int32_t* bstart = (int32_t*)(smem + max_words + kPadWords);  // 4 * max_blocks
+    uint32_t* staged = smem;                                     // lds_words
+    int32_t* bstart = (int32_t*)(smem + lds_words);              // 4 * max_blocks
     uint8_t* bthr = (uint8_t*)(bstart + 4 * max_blocks);         // max_blocks
     uint8_t* bbrc = bthr + max_blocks;                           // max_blocks
     double* nrl = (double*)(((uintptr_t)(bbrc + max_blocks) + 7) & ~(uintptr_t)7);  // 5 * 16
@@ -295,33 +340,27 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
     int st = S1G_PKT_OK;
     if (nq < 0 || nb > max_blocks || nb > kMaxBlocks || (nbytes + 3) / 4 > max_words) st = S1G_PKT_ERR_ARG;
 
-    if (st == S1G_PKT_OK) {
-        // ---- stage the user data as big-endian words, zero past the end
+    PacketWords<ALL> words;
+    {
         const uint8_t* a = ud + offs[p];
         const uintptr_t au = (uintptr_t)a;
-        const uint32_t* al = (const uint32_t*)(au & ~(uintptr_t)3);
-        const int off = (int)(au & 3);
-        const int64_t last_al = ((int64_t)off + nbytes - 1) >> 2;  // last aligned word touching the packet
-        const int nwords = (int)((nbytes + 3) >> 2);
-        for (int i = lane; i < nwords + kPadWords; i += 64) {
-            uint32_t be = 0;
-            if (i < nwords) {
-                uint32_t lo = (i <= last_al) ? al[i] : 0u;
-                uint32_t hi = (off && i + 1 <= last_al) ? al[i + 1] : 0u;
-                uint64_t v = ((uint64_t)hi << 32) | lo;
-                uint32_t le = (uint32_t)(v >> (8 * off));
-                be = __builtin_bswap32(le);
-                const int64_t valid = nbytes - 4 * (int64_t)i;  // bytes of this word inside the packet
-                if (valid < 4) be &= ~(0xFFFFFFFFu >> (8 * valid));
-            }
-            words[i] = be;
-        }
+        words.lds = staged;
+        words.cap = lds_words;
+        words.al = (const uint32_t*)(au & ~(uintptr_t)3);
+        words.off = (int)(au & 3);
+        words.last_al = ((int64_t)words.off + nbytes - 1) >> 2;
+        words.nbytes = nbytes;
+        words.wlim = (nbytes + 3) / 4 + kPadWords;
+    }
+    if (st == S1G_PKT_OK) {
+        // ---- stage the leading words (all of them for packets up to the cap)
+        const int n_stage = (int)min((int64_t)lds_words, words.wlim);
+        for (int i = lane; i < n_stage; i += 64) staged[i] = words.global_word(i);
     }
     __syncthreads();
 
     // ---- phase 1: block structure (uniform control flow)
     const int64_t nbits = 8 * nbytes;
-    const int64_t nwords_staged = (nbytes + 3) / 4 + kPadWords;
     int64_t pos = 0;
     int prev_len = 0;
     uint32_t vb0 = 0, vb1 = 0, vb2 = 0, vb3 = 0;  // BRC of block lane + 64 j
@@ -369,8 +408,8 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
             int lo1, hi1;
             len_bounds(brc, lo1, hi1);
             const int64_t end =
-                brc == 4 ? walk_block<true>(words, nwords_staged, nbits, pos, kLen4, lo1, hi1, nsym, delta, lane, &hdr)
-                         : walk_block<false>(words, nwords_staged, nbits, pos, 0, lo1, hi1, nsym, delta, lane, &hdr);
+                brc == 4 ? walk_block<true, ALL>(words, nbits, pos, kLen4, lo1, hi1, nsym, delta, lane, &hdr)
+                         : walk_block<false, ALL>(words, nbits, pos, 0, lo1, hi1, nsym, delta, lane, &hdr);
             prev_len = (int)(end - pos);
             pos = end;
             if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
@@ -409,9 +448,9 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
         const uint32_t mmask = kMMask[brc];
         const int64_t bp = bstart[ch * max_blocks + b];
         int64_t wi = (bp >> 5) + 2;
-        uint64_t cur = (((uint64_t)words[wi - 2] << 32) | words[wi - 1]) << (bp & 31);
+        uint64_t cur = (((uint64_t)words(wi - 2) << 32) | words(wi - 1)) << (bp & 31);
         int valid = 64 - (int)(bp & 31);
-        uint32_t nxt = wi < nwords_staged ? words[wi] : 0u;
+        uint32_t nxt = words(wi);
         const int q0 = 128 * b;
         const int cnt = (b == nb - 1) ? nq - q0 : 128;
         // out element 2q + (ch & 1), component ch >> 1 (IE: re even, IO: re odd, QE: im even, QO: im odd)
@@ -421,7 +460,7 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
                 cur |= (uint64_t)nxt << (32 - valid);
                 valid += 32;
                 ++wi;
-                nxt = wi < nwords_staged ? words[wi] : 0u;
+                nxt = words(wi);
             }
             const uint32_t v = (uint32_t)(cur >> 53);
             const uint32_t code10 = v & 0x3FFu;
@@ -442,19 +481,26 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
                         int32_t* status) {
     if (n_packets <= 0) return 0;
     const int max_words = (max_bytes + 3) / 4;
+    const int lds_words = std::min(max_words + kPadWords, kLdsWords);
     const int max_blocks = (max_quads + 127) / 128 + 1;
-    size_t lds = sizeof(uint32_t) * (size_t)(max_words + kPadWords) + sizeof(int32_t) * 4 * (size_t)max_blocks +
+    size_t lds = sizeof(uint32_t) * (size_t)lds_words + sizeof(int32_t) * 4 * (size_t)max_blocks +
                  2 * (size_t)max_blocks;
     lds = (lds + 7) & ~(size_t)7;
     lds += sizeof(double) * 80;
     if (lds > 160 * 1024) return -1;
     dim3 grid(n_packets), block(64);
-    if (prec == S1G_F32)
-        hipLaunchKernelGGL(fdbaq_decode_kernel<float>, grid, block, lds, stream, ud, offs, lens, nqs, n_packets,
-                           max_words, max_blocks, (float*)out, ld_out, status);
-    else
-        hipLaunchKernelGGL(fdbaq_decode_kernel<double>, grid, block, lds, stream, ud, offs, lens, nqs, n_packets,
-                           max_words, max_blocks, (double*)out, ld_out, status);
+    const bool all = max_words + kPadWords <= kLdsWords;
+#define S1G_LAUNCH(T, A)                                                                                       \
+    hipLaunchKernelGGL((fdbaq_decode_kernel<T, A>), grid, block, lds, stream, ud, offs, lens, nqs, n_packets, \
+                       max_words, lds_words, max_blocks, (T*)out, ld_out, status)
+    if (prec == S1G_F32) {
+        if (all) S1G_LAUNCH(float, true);
+        else S1G_LAUNCH(float, false);
+    } else {
+        if (all) S1G_LAUNCH(double, true);
+        else S1G_LAUNCH(double, false);
+    }
+#undef S1G_LAUNCH
     return (int)hipGetLastError();
 }
 

@@ -113,6 +113,25 @@ def test_decode_edge_cases(ctx, prec, misalign):
     assert set(st.tolist()) >= {0, 1, 2}
 
 
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_decode_bitexact_large_packets(ctx, prec):
+    """RFI lines compress badly (BRC 4, ~44 KB packets, above the 20 KB the
+    decoder stages in LDS): packets of both kinds in one launch decode
+    bit-exactly."""
+    spec = synth.config1(n_lines=48, n_samples=24000, seed=77)
+    raw = spec.raw()
+    mask = synth.add_rfi(raw, spec, seed=78, frac=0.4, run=(5, 10))
+    burst = synth.encode_burst(spec, raw)
+    lens = np.asarray(burst.ud_lengths)
+    assert lens[mask].min() > 20 * 1024 > lens[~mask].max()
+    db = pipeline.from_synth(burst)
+    out, st = db.decode(ctx, prec)
+    torch.cuda.synchronize()
+    ref, rst = oracle.decode_packets(burst.stream, burst.ud_offsets, burst.ud_lengths, burst.num_quads)
+    assert (rst == 0).all() and (st.cpu().numpy() == 0).all()
+    assert _bits_equal(out.cpu().numpy(), ref, prec)
+
+
 @pytest.mark.parametrize("prec,tol", [(F64, 1e-11), (F32, 2e-6)])
 def test_range_compress(ctx, small_burst, small_chain, prec, tol):
     db = pipeline.from_synth(small_burst)
