@@ -35,7 +35,7 @@ __global__ __launch_bounds__(NT) void af_forward_kernel(const cpx<T>* __restrict
                                                          int L, int N, int W, int npanels, FftPlan plan,
                                                          const cpx<T>* __restrict__ tw, const int32_t* __restrict__ perm,
                                                          const cpx<T>* __restrict__ ph, double* __restrict__ part) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
     __shared__ double red[2][NT / 64];
     const int panel = af_panel(blockIdx.x, npanels);
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(NT) void af_adjoint_kernel(const cpx<T>* __restrict
                                                          const int32_t* __restrict__ perm,
                                                          const cpx<T>* __restrict__ ph, const double* __restrict__ scal,
                                                          double* __restrict__ gpart) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
     const int panel = af_panel(blockIdx.x, npanels);
     if (panel >= npanels) return;

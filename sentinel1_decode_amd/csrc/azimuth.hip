@@ -59,7 +59,7 @@ __global__ __launch_bounds__(NT) void az_fft_fixed_kernel(cpx<T>* __restrict__ d
     constexpr int L = (Rs * ...);
     constexpr int total = L * W;
     static_assert(total <= NT * MAXE, "panel fits the per-thread registers");
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;        // L * W
     cpx<T>* tw = buf + (size_t)L * W;       // L
     int32_t* perm = (int32_t*)(tw + L);     // L
@@ -150,7 +150,7 @@ template <typename T, bool INV, int NT>
 __global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__ data, int64_t ld, int L, int N, int W,
                                                     int npanels, FftPlan plan, const cpx<T>* __restrict__ tw,
                                                     const int32_t* __restrict__ perm, T scale, int sh) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
     const int panel = xcd_panel(blockIdx.x, npanels);
     if (panel >= npanels) return;
@@ -404,7 +404,7 @@ template <typename T, int NT, int EPT>
 __global__ __launch_bounds__(NT) void rcmc_mf_kernel(s1g_azimuth_desc d, const AzColumn* __restrict__ cols,
                                                      const cpx<T>* __restrict__ in, int64_t ld_in,
                                                      cpx<T>* __restrict__ out, int64_t ld_out, int cap) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* tile = (cpx<T>*)smem_raw;
     __shared__ int s_lo, s_hi;
     const int N = d.n_samples, L = d.n_lines;

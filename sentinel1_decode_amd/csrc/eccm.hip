@@ -26,7 +26,7 @@ __global__ __launch_bounds__(NT) void eccm_detect_kernel(const cpx<T>* __restric
                                                          const cpx<T>* __restrict__ twSub,
                                                          const int32_t* __restrict__ permSub,
                                                          T* __restrict__ partial) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
     __shared__ T red[NT / 64];
     const int line = blockIdx.x / r, q = blockIdx.x % r;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
                                                         uint8_t* __restrict__ flags, FftPlan pw,
                                                         const cpx<T>* __restrict__ tw,
                                                         const int32_t* __restrict__ perm) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int line = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
     const int tid = threadIdx.x;
     const cpx<T>* row = in + (int64_t)line * ld_in;

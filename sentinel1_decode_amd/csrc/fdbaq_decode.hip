@@ -307,23 +307,74 @@ __device__ int64_t walk_block(const PacketWords<ALL>& words, int64_t nbits, int6
 }
 
 // BRC of block b (uniform) from the lane-distributed registers.
-__device__ __forceinline__ int brc_of(int b, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3) {
-    const int j = b >> 6, l = b & 63;
-    const uint32_t v = j == 0 ? v0 : j == 1 ? v1 : j == 2 ? v2 : v3;
-    return __builtin_amdgcn_readlane((int)v, l);
+__device__ __forceinline__ int brc_of(int b, uint32_t vb) {
+    return (int)((uint32_t)__builtin_amdgcn_readlane((int)vb, b & 63) >> (3 * (b >> 6))) & 7;
 }
 
+// Decode the symbols of one (channel, block) item from a register bit buffer
+// and reconstruct them (packet.cpp:678-693): for THIDX up to the BRC's
+// limit, m below the flag gives m and m at the flag the simple level;
+// otherwise NRL[brc][m] * SF[THIDX], one FP64 product.  Output element
+// 2q + (ch & 1), component ch >> 1 (IE: re even, IO: re odd, QE: im even,
+// QO: im odd).
 template <typename T, bool ALL>
-__global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
-                                                          const uint64_t* __restrict__ offs,
-                                                          const uint32_t* __restrict__ lens,
-                                                          const int32_t* __restrict__ nqs, int n_packets,
-                                                          int max_words, int lds_words, int max_blocks,
-                                                          T* __restrict__ out, int64_t ld_out,
-                                                          int32_t* __restrict__ status) {
-    extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const int32_t* bstart,
+                                            const uint8_t* bbrc, const uint8_t* bthr, const double* nrl,
+                                            int max_blocks, int nq, int nb, int ch, int b, T* orow) {
+    const int brc = bbrc[b], th = bthr[b];
+    const bool simple = th <= kThidxLimit[brc];
+    const int flag = kMFlag[brc];
+    const double sv = simple ? c_simple[brc][th] : 0.0;
+    const double sf = c_sf[th];
+    const double* nrow = nrl + 16 * brc;
+    const uint64_t tab = len_table(brc);
+    const uint32_t x4 = brc == 4 ? 1u : 0u;
+    const uint64_t mbase = kMBase[brc], mshift = kMShift[brc];
+    const uint32_t mmask = kMMask[brc];
+    const int64_t bp = bstart[ch * max_blocks + b];
+    int64_t wi = (bp >> 5) + 2;
+    uint64_t cur = (((uint64_t)words(wi - 2) << 32) | words(wi - 1)) << (bp & 31);
+    int valid = 64 - (int)(bp & 31);
+    uint32_t nxt = words(wi);
+    const int q0 = 128 * b;
+    const int cnt = (b == nb - 1) ? nq - q0 : 128;
+    T* dst = orow + 2 * (2 * (int64_t)q0 + (ch & 1)) + (ch >> 1);
+    for (int s = 0; s < cnt; ++s) {
+        if (valid < 32) {
+            cur |= (uint64_t)nxt << (32 - valid);
+            valid += 32;
+            ++wi;
+            nxt = words(wi);
+        }
+        const uint32_t v = (uint32_t)(cur >> 53);
+        const uint32_t code10 = v & 0x3FFu;
+        const int clo = fdbaq_clo(code10);
+        const int len = fdbaq_sym_len(code10, clo, tab, x4);
+        const int m = fdbaq_mcode_tab(code10, clo, mbase, mshift, mmask, x4);
+        const double mg = simple ? (m < flag ? (double)m : sv) : __dmul_rn(nrow[m], sf);
+        cur <<= len;
+        valid -= len;
+        dst[4 * (int64_t)s] = (T)((v >> 10) ? -mg : mg);
+    }
+}
+
+// Two wavefronts per packet.  Wave 0 walks the block structure of IE, IO
+// and QE; QE carries the THIDX every channel's reconstruction needs, so at
+// that barrier wave 1 starts decoding the three channels while wave 0 walks
+// QO; after the second barrier both decode QO.  Only workgroup barriers
+// hand work over, and every wave passes each of them once.
+template <typename T, bool ALL>
+__global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
+                                                           const uint64_t* __restrict__ offs,
+                                                           const uint32_t* __restrict__ lens,
+                                                           const int32_t* __restrict__ nqs, int n_packets,
+                                                           int max_words, int lds_words, int max_blocks,
+                                                           T* __restrict__ out, int64_t ld_out,
+                                                           int32_t* __restrict__ status) {
+    extern __shared__ __align__(16) uint32_t smem[];  // base 16-aligned after the static words
+    __shared__ int s_st, s_st_qo;  // status after IE..QE; status of the QO walk
     const int p = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (p >= n_packets) return;
 
     uint32_t* staged = smem;                                     // lds_words
@@ -331,7 +382,7 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
     uint8_t* bthr = (uint8_t*)(bstart + 4 * max_blocks);         // max_blocks
     uint8_t* bbrc = bthr + max_blocks;                           // max_blocks
     double* nrl = (double*)(((uintptr_t)(bbrc + max_blocks) + 7) & ~(uintptr_t)7);  // 5 * 16
-    for (int i = lane; i < 80; i += 64) nrl[i] = c_nrl[i >> 4][i & 15];
+    for (int i = tid; i < 80; i += 128) nrl[i] = c_nrl[i >> 4][i & 15];
 
     const int nq = nqs[p];
     const int64_t nbytes = (int64_t)lens[p];
@@ -355,17 +406,21 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
     if (st == S1G_PKT_OK) {
         // ---- stage the leading words (all of them for packets up to the cap)
         const int n_stage = (int)min((int64_t)lds_words, words.wlim);
-        for (int i = lane; i < n_stage; i += 64) staged[i] = words.global_word(i);
+        for (int i = tid; i < n_stage; i += 128) staged[i] = words.global_word(i);
+    }
+    if (tid == 0) {
+        s_st = st;
+        s_st_qo = S1G_PKT_OK;
     }
     __syncthreads();
 
-    // ---- phase 1: block structure (uniform control flow)
+    // ---- phase 1 (wave 0): block structure, uniform control flow
     const int64_t nbits = 8 * nbytes;
     int64_t pos = 0;
     int prev_len = 0;
-    uint32_t vb0 = 0, vb1 = 0, vb2 = 0, vb3 = 0;  // BRC of block lane + 64 j
-    for (int ch = 0; ch < 4 && st == S1G_PKT_OK; ++ch) {
-        // header field of the channel's first block, read from LDS; later
+    uint32_t vb = 0;  // bits 3j..3j+2: BRC of block lane + 64 j
+    auto walk_channel = [&](int ch) __attribute__((always_inline)) {
+        // header field of the channel's first block from the packet; later
         // ones come out of the walk of the block before
         uint32_t hdr = (ch == 0 || ch == 2) ? peek_bits(words, pos, 8) : 0u;
         for (int b = 0; b < nb && st == S1G_PKT_OK; ++b) {
@@ -381,16 +436,11 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
                     break;
                 }
                 brc = (int)v;
-                const int j = b >> 6;
-                if (lane == (b & 63)) {
-                    if (j == 0) vb0 = v;
-                    else if (j == 1) vb1 = v;
-                    else if (j == 2) vb2 = v;
-                    else vb3 = v;
-                }
+                if (lane == (b & 63)) vb |= v << (3 * (b >> 6));
+                if (lane == 0) bbrc[b] = (uint8_t)v;
                 pos += 3;
             } else {
-                brc = brc_of(b, vb0, vb1, vb2, vb3);
+                brc = brc_of(b, vb);
             }
             if (ch == 2) {
                 if (pos + 8 > nbits) {
@@ -415,65 +465,34 @@ __global__ __launch_bounds__(64) void fdbaq_decode_kernel(const uint8_t* __restr
             if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
         }
         pos = (pos + 15) & ~(int64_t)15;
-    }
-    // BRCs to LDS for phase 2 (bstart/bthr were written by lane 0)
-    for (int j = 0; j < 4; ++j) {
-        const int b = lane + 64 * j;
-        if (b < nb) bbrc[b] = (uint8_t)(j == 0 ? vb0 : j == 1 ? vb1 : j == 2 ? vb2 : vb3);
+    };
+    if (wave == 0 && st == S1G_PKT_OK) {
+        for (int ch = 0; ch < 3 && st == S1G_PKT_OK; ++ch) walk_channel(ch);
+        if (lane == 0 && st != S1G_PKT_OK) s_st = st;
     }
     __syncthreads();
-
-    const int N = 2 * (nq > 0 ? nq : 0);
-    if (st != S1G_PKT_OK) {
-        for (int i = lane; i < 2 * N && i < 2 * ld_out; i += 64) orow[i] = (T)0;
-        if (lane == 0) status[p] = st;
-        return;
-    }
-
-    // ---- phase 2: lanes decode (channel, block) pairs from a register
-    // bit buffer and reconstruct (packet.cpp:678-693): for THIDX up to the
-    // BRC's limit, m below the flag gives m and m at the flag the simple
-    // level; otherwise NRL[brc][m] * SF[THIDX], one FP64 product.
-    for (int item = lane; item < 4 * nb; item += 64) {
-        const int ch = item & 3, b = item >> 2;
-        const int brc = bbrc[b], th = bthr[b];
-        const bool simple = th <= kThidxLimit[brc];
-        const int flag = kMFlag[brc];
-        const double sv = simple ? c_simple[brc][th] : 0.0;
-        const double sf = c_sf[th];
-        const double* nrow = nrl + 16 * brc;
-        const uint64_t tab = len_table(brc);
-        const uint32_t x4 = brc == 4 ? 1u : 0u;
-        const uint64_t mbase = kMBase[brc], mshift = kMShift[brc];
-        const uint32_t mmask = kMMask[brc];
-        const int64_t bp = bstart[ch * max_blocks + b];
-        int64_t wi = (bp >> 5) + 2;
-        uint64_t cur = (((uint64_t)words(wi - 2) << 32) | words(wi - 1)) << (bp & 31);
-        int valid = 64 - (int)(bp & 31);
-        uint32_t nxt = words(wi);
-        const int q0 = 128 * b;
-        const int cnt = (b == nb - 1) ? nq - q0 : 128;
-        // out element 2q + (ch & 1), component ch >> 1 (IE: re even, IO: re odd, QE: im even, QO: im odd)
-        T* dst = orow + 2 * (2 * (int64_t)q0 + (ch & 1)) + (ch >> 1);
-        for (int s = 0; s < cnt; ++s) {
-            if (valid < 32) {
-                cur |= (uint64_t)nxt << (32 - valid);
-                valid += 32;
-                ++wi;
-                nxt = words(wi);
-            }
-            const uint32_t v = (uint32_t)(cur >> 53);
-            const uint32_t code10 = v & 0x3FFu;
-            const int clo = fdbaq_clo(code10);
-            const int len = fdbaq_sym_len(code10, clo, tab, x4);
-            const int m = fdbaq_mcode_tab(code10, clo, mbase, mshift, mmask, x4);
-            const double mg = simple ? (m < flag ? (double)m : sv) : __dmul_rn(nrow[m], sf);
-            cur <<= len;
-            valid -= len;
-            dst[4 * (int64_t)s] = (T)((v >> 10) ? -mg : mg);
+    st = s_st;
+    if (st == S1G_PKT_OK) {
+        if (wave == 0) {
+            walk_channel(3);
+            if (lane == 0 && st != S1G_PKT_OK) s_st_qo = st;
+        } else {
+            // ---- phase 2 (wave 1): IE, IO, QE while wave 0 walks QO
+            for (int item = lane; item < 3 * nb; item += 64)
+                decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, item % 3, item / 3, orow);
         }
     }
-    if (lane == 0) status[p] = S1G_PKT_OK;
+    __syncthreads();
+    st = s_st != S1G_PKT_OK ? s_st : s_st_qo;
+    const int N = 2 * (nq > 0 ? nq : 0);
+    if (st != S1G_PKT_OK) {
+        for (int i = tid; i < 2 * N && i < 2 * ld_out; i += 128) orow[i] = (T)0;
+        if (tid == 0) status[p] = st;
+        return;
+    }
+    // ---- phase 2 (both waves): QO
+    for (int b = tid; b < nb; b += 128) decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, 3, b, orow);
+    if (tid == 0) status[p] = S1G_PKT_OK;
 }
 
 int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
@@ -488,7 +507,7 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     lds = (lds + 7) & ~(size_t)7;
     lds += sizeof(double) * 80;
     if (lds > 160 * 1024) return -1;
-    dim3 grid(n_packets), block(64);
+    dim3 grid(n_packets), block(128);
     const bool all = max_words + kPadWords <= kLdsWords;
 #define S1G_LAUNCH(T, A)                                                                                       \
     hipLaunchKernelGGL((fdbaq_decode_kernel<T, A>), grid, block, lds, stream, ud, offs, lens, nqs, n_packets, \

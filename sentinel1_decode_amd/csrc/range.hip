@@ -101,7 +101,7 @@ __global__ __launch_bounds__(NT) void range_os_kernel(const cpx<T>* __restrict__
                                                       const cpx<T>* __restrict__ tw,
                                                       const int32_t* __restrict__ perm,
                                                       const cpx<T>* __restrict__ h) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
     const int tid = threadIdx.x;
     const int M = plan.n;
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(NT) void range_os3_kernel(const cpx<T>* __restrict_
     static_assert(E == R0 && E == P1 * R1 && E == P2 * R2, "E elements per thread in every pass");
     constexpr int NHI = M / 128;
     constexpr int LDATA = M + M / 16;
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
     cpx<T>* tlo = buf + LDATA;
     cpx<T>* thi = tlo + 128;
