@@ -113,6 +113,95 @@ static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64
                        s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale, sh);
 }
 
+// Persistent form of az_fft_fixed_kernel: a panel fills most of a CU's LDS,
+// so workgroups cannot overlap one another's HBM traffic; instead one
+// workgroup per CU walks its panels and issues the next panel's loads into
+// registers before the LDS passes of the current one.  Panels are dealt so
+// that at every step each XCD holds a contiguous run of them (neighbours
+// share that XCD's L2 lines).
+template <typename T, bool INV, int NT, int W, int MAXE, int... Rs>
+__global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ data, int64_t ld, int N, int npanels,
+                                                         const cpx<T>* __restrict__ tw_g,
+                                                         const int32_t* __restrict__ perm_g, T scale, int sh) {
+    constexpr int L = (Rs * ...);
+    constexpr int total = L * W;
+    static_assert(total <= NT * MAXE, "panel fits the per-thread registers");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* buf = (cpx<T>*)smem_raw;        // L * W
+    cpx<T>* tw = buf + (size_t)L * W;       // L
+    int32_t* perm = (int32_t*)(tw + L);     // L
+    const int tid0 = threadIdx.x;
+    const int G = (int)gridDim.x;           // a multiple of 8
+    const int first = (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+    for (int k = tid0; k < L; k += NT) {
+        tw[k] = tw_g[k];
+        perm[k] = perm_g[k];
+    }
+    cpx<T> r[MAXE];
+    auto load = [&](int panel, int tid) {
+        const int j0 = panel * W;
+        const int w = min(W, N - j0);
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int idx = tid + NT * i;
+            const int e = idx / W, v = idx % W;
+            r[i] = mk((T)0, (T)0);
+            if (idx < total && v < w) r[i] = data[(int64_t)e * ld + j0 + v];
+        }
+    };
+    if (first < npanels) load(first, tid0);
+    for (int panel = first; panel < npanels; panel += G) {
+        // The index arithmetic of the loads, passes and stores depends only
+        // on the thread: hidden from the optimiser per panel so that it is
+        // recomputed rather than hoisted out of the loop and kept live.
+        int tid = tid0;
+        asm volatile("" : "+v"(tid));
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int idx = tid + NT * i;
+            if (idx < total) buf[idx] = r[i];
+        }
+        __syncthreads();
+        if (panel + G < npanels) load(panel + G, tid);  // in flight during the passes
+        DifSeq<T, INV, Rs...>::run(buf, L, 1, W, 1, W, tw, tid, NT);
+        const int j0 = panel * W;
+        const int w = min(W, N - j0);
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int idx = tid + NT * i;
+            const int k = idx / W, v = idx % W;
+            if (idx < total && v < w) {
+                int src = k + sh;
+                if (src >= L) src -= L;
+                const cpx<T> y = buf[perm[src] * W + v];
+                data[(int64_t)k * ld + j0 + v] = INV ? scl(y, scale) : y;
+            }
+        }
+        __syncthreads();  // the panel's LDS is rewritten next
+    }
+}
+
+// Workgroups of the persistent kernel: one per CU (the LDS allows no more),
+// a multiple of 8 so every XCD gets the same count.
+static int pipe_workgroups(int npanels) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        ncu = 256;
+    const int g = std::max(8, (ncu / 8) * 8);
+    return std::min(g, ((npanels + 7) / 8) * 8);
+}
+
+template <typename T, bool INV, int W, int... Rs>
+static void launch_az_pipe(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
+    constexpr int NT = 512;
+    constexpr int L = (Rs * ...);
+    constexpr int MAXE = (L * W + NT - 1) / NT;
+    const int npanels = (N + W - 1) / W;
+    const size_t lds = (size_t)L * W * sizeof(cpx<T>) + (size_t)L * (sizeof(cpx<T>) + sizeof(int32_t));
+    hipLaunchKernelGGL((az_fft_pipe_kernel<T, INV, NT, W, MAXE, Rs...>), dim3((unsigned)pipe_workgroups(npanels)),
+                       dim3(NT), lds, s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale, sh);
+}
+
 // Whether a plan's radix sequence is one compiled into az_fft_fixed_kernel.
 static int fixed_sequence(const FftPlan& p) {
     auto is = [&](std::initializer_list<int> rs) {
@@ -129,21 +218,24 @@ static int fixed_sequence(const FftPlan& p) {
 }
 
 // Plans with a compiled radix sequence (the IW1/IW2/IW3 burst lengths).
-template <typename T, bool INV, int W>
+template <typename T, bool INV, int W, bool PIPE>
 static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
-    const FftPlan& p = fl.plan;
-    auto is = [&](std::initializer_list<int> rs) {
-        if ((int)rs.size() != p.npass) return false;
-        int k = 0;
-        for (int r : rs)
-            if (p.radix[k++] != r) return false;
+    switch (fixed_sequence(fl.plan)) {
+    case 1:
+        if constexpr (PIPE) launch_az_pipe<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale, sh);
+        else launch_az_fixed<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale, sh);
         return true;
-    };
-    if (is({37, 5, 8})) launch_az_fixed<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale, sh);
-    else if (is({71, 5, 4})) launch_az_fixed<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale, sh);
-    else if (is({19, 5, 8, 2})) launch_az_fixed<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale, sh);
-    else return false;
-    return true;
+    case 2:
+        if constexpr (PIPE) launch_az_pipe<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale, sh);
+        else launch_az_fixed<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale, sh);
+        return true;
+    case 3:
+        if constexpr (PIPE) launch_az_pipe<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale, sh);
+        else launch_az_fixed<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale, sh);
+        return true;
+    default:
+        return false;
+    }
 }
 
 template <typename T, bool INV, int NT>
@@ -222,26 +314,40 @@ static int launch_az_generic(hipStream_t s, int prec, const DevFft& fl, void* da
     return (int)hipGetLastError();
 }
 
-int azimuth_fft_variants(const DevFft& fl) { return fixed_sequence(fl.plan) ? 3 : 1; }
+int azimuth_fft_variants(const DevFft& fl) { return fixed_sequence(fl.plan) ? 4 : 1; }
 
 // Variants: with a compiled radix sequence 0 = 64-byte panels, 1 = 32-byte
-// panels (twice the workgroups per CU), 2 = the generic kernel; otherwise
-// only the generic kernel.
+// panels (twice the workgroups per CU), 2 = the generic kernel, 3 = 64-byte
+// panels, persistent with the next panel's loads overlapping the passes;
+// otherwise only the generic kernel.
 int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
                        int inverse, int variant, int shift) {
     const int sh = (!inverse && shift) ? L - L / 2 : 0;
-    if (fixed_sequence(fl.plan) && variant < 2) {
-        const bool wide = variant == 0;
+    if (fixed_sequence(fl.plan) && variant != 2) {
         if (prec == S1G_F32) {
             const float sc = (float)(1.0 / (double)L);
             cpx<float>* d = (cpx<float>*)data;
-            if (wide) inverse ? az_fixed<float, true, 8>(s, fl, d, ld, N, sc, sh) : az_fixed<float, false, 8>(s, fl, d, ld, N, 1.0f, sh);
-            else inverse ? az_fixed<float, true, 4>(s, fl, d, ld, N, sc, sh) : az_fixed<float, false, 4>(s, fl, d, ld, N, 1.0f, sh);
+            if (variant == 3)
+                inverse ? az_fixed<float, true, 8, true>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<float, false, 8, true>(s, fl, d, ld, N, 1.0f, sh);
+            else if (variant == 1)
+                inverse ? az_fixed<float, true, 4, false>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<float, false, 4, false>(s, fl, d, ld, N, 1.0f, sh);
+            else
+                inverse ? az_fixed<float, true, 8, false>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<float, false, 8, false>(s, fl, d, ld, N, 1.0f, sh);
         } else {
             const double sc = 1.0 / (double)L;
             cpx<double>* d = (cpx<double>*)data;
-            if (wide) inverse ? az_fixed<double, true, 4>(s, fl, d, ld, N, sc, sh) : az_fixed<double, false, 4>(s, fl, d, ld, N, 1.0, sh);
-            else inverse ? az_fixed<double, true, 2>(s, fl, d, ld, N, sc, sh) : az_fixed<double, false, 2>(s, fl, d, ld, N, 1.0, sh);
+            if (variant == 3)
+                inverse ? az_fixed<double, true, 4, true>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<double, false, 4, true>(s, fl, d, ld, N, 1.0, sh);
+            else if (variant == 1)
+                inverse ? az_fixed<double, true, 2, false>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<double, false, 2, false>(s, fl, d, ld, N, 1.0, sh);
+            else
+                inverse ? az_fixed<double, true, 4, false>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<double, false, 4, false>(s, fl, d, ld, N, 1.0, sh);
         }
         return (int)hipGetLastError();
     }
@@ -267,6 +373,7 @@ __global__ void az_columns_kernel(s1g_azimuth_desc d, AzColumn* __restrict__ col
     c.amp = 1.0 / c.sq;
     c.bt = c.b * inv2pi;
     c.ct = c.c * inv2pi;
+    c.btf = c.bt - rint(c.bt);
     cols[j] = c;
 }
 
@@ -496,6 +603,210 @@ __global__ __launch_bounds__(NT) void rcmc_mf_kernel(s1g_azimuth_desc d, const A
     }
 }
 
+// FP32 chain, runs of consecutive samples.  For u = lambda^2 f^2 / (4 v^2)
+// up to kSeriesU the geometry takes the series of 1/D - 1 and 1 - D
+// (D = sqrt(1 - u)) to u^5, whose truncation is below 5e-16 relative; the
+// phase bt D + ct f is formed as (bt - rint(bt)) - bt (1 - D) + ct f.
+constexpr double kSeriesU = 1e-3;
+
+__device__ __forceinline__ void geo32_series(const AzColumn& col, double fa, int j, bool rcmc, int& n0, double& fr,
+                                             float& c, float& s, float& amp) {
+    const double f = fa + col.fdc;
+    const double u = col.a * (f * f);
+    double idm1, omd;  // 1/D - 1 and 1 - D
+    if (u <= kSeriesU) {
+        idm1 = u * (0.5 + u * (0.375 + u * (0.3125 + u * (0.2734375 + u * 0.24609375))));
+        omd = u * (0.5 + u * (0.125 + u * (0.0625 + u * (0.0390625 + u * 0.02734375))));
+    } else {
+        const double D = sqrt(1.0 - u);
+        omd = u / (1.0 + D);
+        idm1 = omd / D;
+    }
+    const double shift = col.rdr * idm1;
+    const double fl = floor(shift);
+    n0 = rcmc ? j + (int)fl : j;
+    fr = rcmc ? shift - fl : 0.0;
+    const double pt = (col.btf - col.bt * omd) + col.ct * f;
+    const float pr = (float)(pt - rint(pt));
+    s = __builtin_amdgcn_sinf(pr);
+    c = __builtin_amdgcn_cosf(pr);
+    amp = (float)col.amp;
+}
+
+// Largest spread of the fractional offset over one run for which the
+// weights are interpolated: the windowed sinc has |w''| < 20, so the linear
+// interpolation error stays below 20 * (2.5e-4)^2 / 8 = 1.6e-7.
+constexpr double kInterpSpan = 2.5e-4;
+
+// One workgroup per (Doppler row i, NT runs of EPT consecutive samples).
+// Along a row the migration changes by at most u/2 per sample, so within a
+// run the tap set usually moves with the sample (n0 = j + const) and the
+// fractional offset by less than kInterpSpan: the run then evaluates the
+// windowed-sinc weights exactly at its first and last samples, interpolates
+// them linearly in the offset for the others, and reads its 2*8 + EPT taps
+// from the LDS tile once.  Other runs take the per-sample path.  The
+// geometry (column table reads) and the matched-filter epilogue (stores)
+// run with consecutive lanes on consecutive samples; LDS hands n0 and the
+// offset to the runs and the runs' sums back (EPT odd keeps the lanes' runs
+// on distinct LDS banks).
+template <int NT, int EPT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void rcmc32_run_kernel(s1g_azimuth_desc d, const AzColumn* __restrict__ cols,
+                                                        const cpx<float>* __restrict__ in, int64_t ld_in,
+                                                        cpx<float>* __restrict__ out, int64_t ld_out, int cap) {
+    constexpr int NW = 2 * kTaps + EPT;
+    constexpr int PER = NT * EPT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float* s_fr = (float*)smem_raw;                    // PER
+    int* s_n0 = (int*)(s_fr + PER);                    // PER
+    cpx<float>* tile = (cpx<float>*)(s_n0 + PER);      // cap (>= PER): taps, then the runs' sums
+    __shared__ int s_lo[NT / 64], s_hi[NT / 64];
+    const int N = d.n_samples, L = d.n_lines;
+    const int i = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int j0 = blockIdx.x * PER;
+    const bool rcmc = d.apply_rcmc != 0;
+    double fa;
+    {
+#pragma clang fp contract(off)
+        const double start = -d.prf / 2, end = d.prf / 2;
+        const double step = (end - start) / (L - 1);
+        fa = start + i * step;
+    }
+    const cpx<float>* row = in + (int64_t)i * ld_in;
+    // ---- geometry, consecutive lanes on consecutive samples
+    float cs[EPT], sn[EPT], amp[EPT];
+    int lo = 1 << 30, hi = -(1 << 30);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int q = e * NT + tid;
+        const int j = j0 + q;
+        int n0 = j;
+        double fr = 0.0;
+        cs[e] = sn[e] = amp[e] = 0.0f;
+        if (j < N) {
+            geo32_series(cols[j], fa, j, rcmc, n0, fr, cs[e], sn[e], amp[e]);
+            lo = min(lo, n0 - kTaps);
+            hi = max(hi, n0 + kTaps);
+        }
+        s_fr[q] = (float)fr;
+        s_n0[q] = n0;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    if ((tid & 63) == 0) {
+        s_lo[tid >> 6] = lo;
+        s_hi[tid >> 6] = hi;
+    }
+    __syncthreads();
+    int tlo = s_lo[0], thi = s_hi[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) {
+        tlo = min(tlo, s_lo[w]);
+        thi = max(thi, s_hi[w]);
+    }
+    const bool use_tile = (thi - tlo + 1) <= cap;
+    if (use_tile)
+        for (int k = tlo + tid; k <= thi; k += NT) tile[k - tlo] = (k >= 0 && k < N) ? row[k] : mk(0.0f, 0.0f);
+    __syncthreads();
+    // ---- the run of samples j0 + tid * EPT + e
+    const int r0 = tid * EPT;
+    const int js = j0 + r0;
+    cpx<float> acc[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) acc[e] = mk(0.0f, 0.0f);
+    if (js < N) {
+        int n0[EPT];
+        float fr[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            n0[e] = s_n0[r0 + e];
+            fr[e] = s_fr[r0 + e];
+        }
+        bool fast = use_tile && rcmc && js + EPT <= N && fabsf(fr[EPT - 1] - fr[0]) <= (float)kInterpSpan;
+#pragma unroll
+        for (int e = 1; e < EPT; ++e) fast = fast && n0[e] == n0[0] + e;
+        if (fast) {
+            float w0[2 * kTaps + 1], dw[2 * kTaps + 1];
+            sinc_weights(fr[0], w0);
+            sinc_weights(fr[EPT - 1], dw);
+#pragma unroll
+            for (int k = 0; k <= 2 * kTaps; ++k) dw[k] -= w0[k];
+            const float dfr = fr[EPT - 1] - fr[0];
+            const float inv = dfr != 0.0f ? 1.0f / dfr : 0.0f;
+            const cpx<float>* tp = tile + (n0[0] - kTaps - tlo);
+            cpx<float> x[NW];
+#pragma unroll
+            for (int m = 0; m < NW; ++m) x[m] = tp[m];
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                const float t = (fr[e] - fr[0]) * inv;
+#pragma unroll
+                for (int k = 0; k <= 2 * kTaps; ++k) {
+                    const float w = fmaf(t, dw[k], w0[k]);
+                    acc[e].x += x[e + k].x * w;
+                    acc[e].y += x[e + k].y * w;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                const int j = js + e;
+                if (j >= N) continue;
+                if (!rcmc) {
+                    acc[e] = row[j];
+                    continue;
+                }
+                float w[2 * kTaps + 1];
+                sinc_weights(fr[e], w);
+                if (use_tile) {
+                    const cpx<float>* tp = tile + (n0[e] - kTaps - tlo);
+#pragma unroll
+                    for (int k = 0; k <= 2 * kTaps; ++k) {
+                        acc[e].x += tp[k].x * w[k];
+                        acc[e].y += tp[k].y * w[k];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k <= 2 * kTaps; ++k) {
+                        const int idx = n0[e] - kTaps + k;
+                        if (idx < 0 || idx >= N) continue;
+                        const cpx<float> xv = row[idx];
+                        acc[e].x += xv.x * w[k];
+                        acc[e].y += xv.y * w[k];
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();  // every run has read its taps: the tile now takes the sums
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) tile[r0 + e] = acc[e];
+    __syncthreads();
+    // ---- amplitude and matched filter, consecutive lanes on consecutive samples
+    cpx<float>* orow = out + (int64_t)i * ld_out;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int q = e * NT + tid;
+        const int j = j0 + q;
+        if (j < N) orow[j] = cmul(scl(tile[q], amp[e]), mk(cs[e], sn[e]));
+    }
+}
+
+template <int NT, int EPT>
+static void launch_rcmc32_run(hipStream_t s, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
+                              int64_t ld_in, void* out, int64_t ld_out) {
+    const int per = NT * EPT;
+    const int cap = per + 2 * kTaps + 256;  // migration spans beyond 256 samples read global memory
+    const size_t lds = (sizeof(float) + sizeof(int)) * (size_t)per + sizeof(cpx<float>) * (size_t)cap;
+    dim3 grid((unsigned)((d.n_samples + per - 1) / per), (unsigned)d.n_lines);
+    hipLaunchKernelGGL((rcmc32_run_kernel<NT, EPT>), grid, dim3(NT), lds, s, d, cols, (const cpx<float>*)in, ld_in,
+                       (cpx<float>*)out, ld_out, cap);
+}
+
+int rcmc_variants(int prec) { return prec == S1G_F32 ? 5 : 3; }
+
 template <typename T, int NT, int EPT>
 static void launch_rcmc_t(hipStream_t s, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
                           int64_t ld_in, void* out, int64_t ld_out) {
@@ -506,11 +817,15 @@ static void launch_rcmc_t(hipStream_t s, const s1g_azimuth_desc& d, const AzColu
                        (const cpx<T>*)in, ld_in, (cpx<T>*)out, ld_out, cap);
 }
 
-// Variants (threads x samples per thread): 0 = 256 x 8, 1 = 256 x 4, 2 = 512 x 4.
+// Variants (threads x samples per thread): 0 = 256 x 8, 1 = 256 x 4, 2 = 512 x 4;
+// FP32 also runs of 9 consecutive samples per thread (rcmc32_run_kernel):
+// 3 = 256 threads, 4 = 128 threads.
 int launch_rcmc_mf(hipStream_t s, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
                    int64_t ld_in, void* out, int64_t ld_out, int variant) {
     if (prec == S1G_F32) {
-        if (variant == 1) launch_rcmc_t<float, 256, 4>(s, d, cols, in, ld_in, out, ld_out);
+        if (variant == 3) launch_rcmc32_run<256, 9>(s, d, cols, in, ld_in, out, ld_out);
+        else if (variant == 4) launch_rcmc32_run<128, 9>(s, d, cols, in, ld_in, out, ld_out);
+        else if (variant == 1) launch_rcmc_t<float, 256, 4>(s, d, cols, in, ld_in, out, ld_out);
         else if (variant == 2) launch_rcmc_t<float, 512, 4>(s, d, cols, in, ld_in, out, ld_out);
         else launch_rcmc_t<float, 256, 8>(s, d, cols, in, ld_in, out, ld_out);
     } else {

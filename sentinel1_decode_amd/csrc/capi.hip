@@ -405,7 +405,7 @@ int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int
                             "rcmc/matched filter");
     };
     const int v = choose_variant(c, TuneKey{S1G_STAGE_RCMC_MF, prec, d.n_lines, d.n_samples, d.apply_rcmc},
-                                 kRcmcVariants, run);
+                                 rcmc_variants(prec), run);
     StageTimer tm(c, S1G_STAGE_RCMC_MF);
     return run(v);
 }

@@ -69,6 +69,10 @@ __constant__ double c_sf[256] = {
 // bits beyond the last checked position, plus the 4-word window.
 constexpr int kPadWords = 72;
 
+// Bit positions are 32-bit: user data up to 2^27 bytes (a Sentinel-1 space
+// packet carries at most 65534).
+constexpr int kMaxPacketBytes = 1 << 27;
+
 // Bits a lane's speculative walk starts before its slice, so that it has
 // usually fallen into step with the true symbol boundaries on arrival.
 // kOverlap + 32 (the widest slice) + 11 (the longest symbol) <= 64 keeps
@@ -94,31 +98,35 @@ struct PacketWords {
     int cap;
     const uint32_t* al;  // 4-byte aligned, at or before the packet's first byte
     int off;             // packet start - al, bytes
-    int64_t last_al;     // last aligned word touching the packet
-    int64_t nbytes;
-    int64_t wlim;
+    int last_al;         // last aligned word touching the packet
+    int nbytes;
+    int wlim;
 
-    __device__ __forceinline__ uint32_t global_word(int64_t i) const {
+    __device__ __forceinline__ uint32_t global_word(int i) const {
         if (4 * i >= nbytes) return 0u;
         const uint32_t lo = (i <= last_al) ? al[i] : 0u;
         const uint32_t hi = (off && i + 1 <= last_al) ? al[i + 1] : 0u;
         const uint64_t v = ((uint64_t)hi << 32) | lo;
         uint32_t be = __builtin_bswap32((uint32_t)(v >> (8 * off)));
-        const int64_t valid = nbytes - 4 * i;  // bytes of this word inside the packet
+        const int valid = nbytes - 4 * i;  // bytes of this word inside the packet
         if (valid < 4) be &= ~(0xFFFFFFFFu >> (8 * valid));
         return be;
     }
-    __device__ __forceinline__ uint32_t operator()(int64_t i) const {
+    // With the whole packet staged (ALL) the reads need no bounds test: the
+    // walk reads at most 67 words past the data (lane 63 of a block starting
+    // at the last bit, slices of at most 32 bits, a 5-word window) and the
+    // kPadWords staged zero words cover that.
+    __device__ __forceinline__ uint32_t operator()(int i) const {
+        if constexpr (ALL) return lds[i];
         if (i >= wlim) return 0u;
-        if constexpr (ALL) return lds[i];  // the whole packet (and padding) is staged
         return i < cap ? lds[i] : global_word(i);
     }
 };
 
 // `n` (1..32) bits starting at bit `pos`.
 template <bool ALL>
-__device__ __forceinline__ uint32_t peek_bits(const PacketWords<ALL>& w, int64_t pos, int n) {
-    const int64_t wi = pos >> 5;
+__device__ __forceinline__ uint32_t peek_bits(const PacketWords<ALL>& w, int pos, int n) {
+    const int wi = pos >> 5;
     const int sh = (int)(pos & 31);
     uint64_t v = ((uint64_t)w(wi) << 32) | (uint64_t)w(wi + 1);
     return (uint32_t)((v << sh) >> (64 - n));
@@ -209,19 +217,19 @@ __device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
 // THIDX field).  Reads stay below word `wlim`; once `start` passes `nbits`
 // the block cannot end inside the packet and the walk stops there.
 template <bool B4, bool ALL>
-__device__ int64_t walk_block(const PacketWords<ALL>& words, int64_t nbits, int64_t start, uint64_t tab, int lo1,
+__device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, uint64_t tab, int lo1,
                               int hi1, int nsym, int delta, int lane, uint32_t* hdr) {
     for (;;) {
         if (start > nbits) {
             *hdr = 0;
             return start;
         }
-        const int64_t s = start + (int64_t)lane * delta;
+        const int s = start + lane * delta;
         // BRC 4 codes resynchronise more slowly: a longer lead-in, within the
         // 64-bit window (lead-in + delta + 11 <= 64)
         const int lead = B4 ? min(kOverlapB4, 53 - delta) : kOverlap;
-        const int64_t org = max(s - lead, start);
-        const int64_t w0 = org >> 5;
+        const int org = max(s - lead, start);
+        const int w0 = org >> 5;
         Win5 w;
         w.w0 = words(w0);
         w.w1 = words(w0 + 1);
@@ -299,10 +307,10 @@ __device__ int64_t walk_block(const PacketWords<ALL>& words, int64_t nbits, int6
             }
             endrel = __builtin_amdgcn_readlane(endrel, ls);
             *hdr = (uint32_t)__builtin_amdgcn_readlane((int)h, ls);
-            return start + (int64_t)ls * delta + endrel;
+            return start + ls * delta + endrel;
         }
         nsym -= total;
-        start += (int64_t)63 * delta + __builtin_amdgcn_readlane(ex, 63);
+        start += 63 * delta + __builtin_amdgcn_readlane(ex, 63);
     }
 }
 
@@ -331,8 +339,8 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
     const uint32_t x4 = brc == 4 ? 1u : 0u;
     const uint64_t mbase = kMBase[brc], mshift = kMShift[brc];
     const uint32_t mmask = kMMask[brc];
-    const int64_t bp = bstart[ch * max_blocks + b];
-    int64_t wi = (bp >> 5) + 2;
+    const int bp = bstart[ch * max_blocks + b];
+    int wi = (bp >> 5) + 2;
     uint64_t cur = (((uint64_t)words(wi - 2) << 32) | words(wi - 1)) << (bp & 31);
     int valid = 64 - (int)(bp & 31);
     uint32_t nxt = words(wi);
@@ -385,11 +393,14 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     for (int i = tid; i < 80; i += 128) nrl[i] = c_nrl[i >> 4][i & 15];
 
     const int nq = nqs[p];
-    const int64_t nbytes = (int64_t)lens[p];
+    const uint32_t nbytes_u = lens[p];
+    const int nbytes = (int)min(nbytes_u, (uint32_t)kMaxPacketBytes);
     const int nb = nq > 0 ? (nq + 127) / 128 : 0;
     T* orow = out + (int64_t)2 * ld_out * p;
     int st = S1G_PKT_OK;
-    if (nq < 0 || nb > max_blocks || nb > kMaxBlocks || (nbytes + 3) / 4 > max_words) st = S1G_PKT_ERR_ARG;
+    if (nq < 0 || nb > max_blocks || nb > kMaxBlocks || nbytes_u > (uint32_t)kMaxPacketBytes ||
+        (nbytes + 3) / 4 > max_words)
+        st = S1G_PKT_ERR_ARG;
 
     PacketWords<ALL> words;
     {
@@ -399,13 +410,13 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
         words.cap = lds_words;
         words.al = (const uint32_t*)(au & ~(uintptr_t)3);
         words.off = (int)(au & 3);
-        words.last_al = ((int64_t)words.off + nbytes - 1) >> 2;
+        words.last_al = (words.off + nbytes - 1) >> 2;
         words.nbytes = nbytes;
         words.wlim = (nbytes + 3) / 4 + kPadWords;
     }
     if (st == S1G_PKT_OK) {
         // ---- stage the leading words (all of them for packets up to the cap)
-        const int n_stage = (int)min((int64_t)lds_words, words.wlim);
+        const int n_stage = min(lds_words, words.wlim);
         for (int i = tid; i < n_stage; i += 128) staged[i] = words.global_word(i);
     }
     if (tid == 0) {
@@ -415,8 +426,8 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     __syncthreads();
 
     // ---- phase 1 (wave 0): block structure, uniform control flow
-    const int64_t nbits = 8 * nbytes;
-    int64_t pos = 0;
+    const int nbits = 8 * nbytes;
+    int pos = 0;
     int prev_len = 0;
     uint32_t vb = 0;  // bits 3j..3j+2: BRC of block lane + 64 j
     auto walk_channel = [&](int ch) __attribute__((always_inline)) {
@@ -457,14 +468,14 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
             const int delta = min(max((est + 63) / 64 + 1, 2), 32);
             int lo1, hi1;
             len_bounds(brc, lo1, hi1);
-            const int64_t end =
+            const int end =
                 brc == 4 ? walk_block<true, ALL>(words, nbits, pos, kLen4, lo1, hi1, nsym, delta, lane, &hdr)
                          : walk_block<false, ALL>(words, nbits, pos, 0, lo1, hi1, nsym, delta, lane, &hdr);
             prev_len = (int)(end - pos);
             pos = end;
             if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
         }
-        pos = (pos + 15) & ~(int64_t)15;
+        pos = (pos + 15) & ~15;
     };
     if (wave == 0 && st == S1G_PKT_OK) {
         for (int ch = 0; ch < 3 && st == S1G_PKT_OK; ++ch) walk_channel(ch);

@@ -38,6 +38,7 @@ struct AzColumn {
     double sq;    // sqrt((R_ref / R_j)^3), the reference's divisor
     double bt;    // b / (2 pi): phase coefficients in turns
     double ct;    // c / (2 pi)
+    double btf;   // bt - rint(bt)
 };
 
 int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
@@ -60,7 +61,7 @@ int launch_range_os(hipStream_t stream, int prec, const DevFft& fm, const void* 
 // Launch variants (candidates of the autotuner): variant counts and the
 // launch with a given variant.
 int azimuth_fft_variants(const DevFft& fl);
-constexpr int kRcmcVariants = 3;
+int rcmc_variants(int prec);
 // shift = 1: fftshift after the forward transform (the azimuth chain);
 // shift = 0: plain transform.  The inverse is scaled by 1/n_lines.
 int launch_azimuth_fft(hipStream_t stream, int prec, const DevFft& fl, void* data, int64_t ld, int n_lines,

@@ -65,7 +65,7 @@ def test_azimuth_variants(vctx, L, inverse):
     for prec, dt, tol in ((F64, torch.complex128, 1e-13), (F32, torch.complex64, 2e-6)):
         x0 = torch.from_numpy(a).to(dt).cuda()
         n = _variant_count(vctx, stage, lambda: vctx.azimuth_fft(x0.clone(), inverse=inverse))
-        assert n == 3
+        assert n == 4
         for v in range(n):
             vctx.force_variant(stage, v)
             x = x0.clone()
@@ -73,6 +73,25 @@ def test_azimuth_variants(vctx, L, inverse):
             torch.cuda.synchronize()
             assert vctx.last_variant(stage) == (v, n)
             assert rel_l2(x.cpu().numpy(), ref) < tol, (prec, v)
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_azimuth_persistent_many_panels(vctx, inverse):
+    """The persistent variant (3) walks several panels per workgroup, the
+    last one partial: 4500 columns = 563 panels of 8 (FP32) / 1125 of 4 (FP64)."""
+    L, N = 1480, 4500
+    rng = np.random.default_rng(11 + inverse)
+    a = rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N))
+    ref = oracle.azimuth_ifft(a) if inverse else oracle.azimuth_fft(a)
+    stage = "az_ifft" if inverse else "az_fft"
+    for prec, dt, tol in ((F64, torch.complex128, 1e-13), (F32, torch.complex64, 2e-6)):
+        vctx.force_variant(stage, 3)
+        x = torch.from_numpy(a).to(dt).cuda()
+        vctx.azimuth_fft(x, inverse=inverse)
+        torch.cuda.synchronize()
+        assert vctx.last_variant(stage) == (3, 4)
+        assert rel_l2(x.cpu().numpy(), ref) < tol, prec
+    vctx.force_variant(stage, -1)
 
 
 @pytest.fixture(scope="module")
@@ -89,13 +108,38 @@ def test_rcmc_variants(vctx, rd_case, prec, tol):
     dt = torch.complex128 if prec == F64 else torch.complex64
     x = torch.from_numpy(ch["rd"]).to(dt).cuda()
     n = _variant_count(vctx, "rcmc_mf", lambda: vctx.rcmc_matched_filter(x, db.azimuth_desc()))
-    assert n == 3
+    assert n == (5 if prec == F32 else 3)
     for v in range(n):
         vctx.force_variant("rcmc_mf", v)
         y = vctx.rcmc_matched_filter(x, db.azimuth_desc())
         torch.cuda.synchronize()
         assert vctx.last_variant("rcmc_mf") == (v, n)
         assert rel_l2(y.cpu().numpy(), ch["mf"]) < tol, v
+
+
+@pytest.mark.parametrize("lam_scale", [1.0, 4.0, 12.0, 60.0])
+def test_rcmc_runs_against_fp64(vctx, rd_case, lam_scale):
+    """FP32 run kernels (variants 3, 4) against the FP64 path on the same input.
+    A longer wavelength raises u = lambda^2 f^2 / (4 v^2) (about 1e-5 at
+    the band edge here): x4 spreads the fractional offsets of the outer
+    rows' runs past the interpolation limit (per-sample weights), x12 and
+    x60 take u past the series limit (square-root geometry; at x60 the
+    migration also outgrows the LDS tile)."""
+    burst, ch = rd_case
+    db = pipeline.from_synth(burst)
+    desc = db.azimuth_desc()
+    desc.wavelength *= lam_scale
+    x64 = torch.from_numpy(ch["rd"]).cuda()
+    vctx.force_variant("rcmc_mf", 0)
+    ref = vctx.rcmc_matched_filter(x64, desc).cpu().numpy()
+    x32 = x64.to(torch.complex64)
+    for v in (0, 3, 4):
+        vctx.force_variant("rcmc_mf", v)
+        y = vctx.rcmc_matched_filter(x32, desc)
+        torch.cuda.synchronize()
+        assert vctx.last_variant("rcmc_mf") == (v, 5)
+        assert rel_l2(y.cpu().numpy(), ref) < 2e-6, (lam_scale, v)
+    vctx.force_variant("rcmc_mf", -1)
 
 
 def test_autotuned_chain_and_forced_graph(vctx):
