@@ -118,7 +118,12 @@ static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64
 // workgroup per CU walks its panels and issues the next panel's loads into
 // registers before the LDS passes of the current one.  Panels are dealt so
 // that at every step each XCD holds a contiguous run of them (neighbours
-// share that XCD's L2 lines).
+// share that XCD's L2 lines).  Loads and stores are raw buffer accesses
+// whose masked elements carry an out-of-range offset: with no branch around
+// them every thread issues exactly MAXE loads then MAXE stores per panel, so
+// the wait for a panel's prefetched registers counts past the previous
+// panel's stores, which keep draining while the next panel is transformed.
+// The array must span less than 2 GiB (checked by the launcher).
 template <typename T, bool INV, int NT, int W, int MAXE, int... Rs>
 __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ data, int64_t ld, int N, int npanels,
                                                          const cpx<T>* __restrict__ tw_g,
@@ -133,36 +138,47 @@ __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ da
     const int tid0 = threadIdx.x;
     const int G = (int)gridDim.x;           // a multiple of 8
     const int first = (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+    const uint32_t bytes = (uint32_t)(((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>));
+    const auto rsrc = buf_rsrc(data, bytes);
     for (int k = tid0; k < L; k += NT) {
         tw[k] = tw_g[k];
         perm[k] = perm_g[k];
     }
     cpx<T> r[MAXE];
+    // element offsets of a panel (kBufSkip past the array, the panel or the
+    // last panel's columns)
     auto load = [&](int panel, int tid) {
         const int j0 = panel * W;
-        const int w = min(W, N - j0);
+        const int w = panel < npanels ? min(W, N - j0) : 0;
 #pragma unroll
         for (int i = 0; i < MAXE; ++i) {
             const int idx = tid + NT * i;
             const int e = idx / W, v = idx % W;
-            r[i] = mk((T)0, (T)0);
-            if (idx < total && v < w) r[i] = data[(int64_t)e * ld + j0 + v];
+            const bool ok = idx < total && v < w;
+            const uint32_t off = ok ? (uint32_t)(((int64_t)e * ld + j0 + v) * (int64_t)sizeof(cpx<T>)) : kBufSkip;
+            r[i] = buf_load<T>(rsrc, off);
         }
     };
-    if (first < npanels) load(first, tid0);
+    auto fill = [&](int tid) {
+#pragma unroll
+        for (int i = 0; i < MAXE; ++i) {
+            const int idx = tid + NT * i;
+            if (idx < total) buf[idx] = r[i];
+        }
+    };
+    load(first, tid0);
+    fill(tid0);
+    __syncthreads();
+    // Each step: prefetch the next panel, transform and store this one, then
+    // move the prefetched registers into LDS.  The move waits for the
+    // prefetch only: the stores issued after it are left to drain.
     for (int panel = first; panel < npanels; panel += G) {
         // The index arithmetic of the loads, passes and stores depends only
         // on the thread: hidden from the optimiser per panel so that it is
         // recomputed rather than hoisted out of the loop and kept live.
         int tid = tid0;
         asm volatile("" : "+v"(tid));
-#pragma unroll
-        for (int i = 0; i < MAXE; ++i) {
-            const int idx = tid + NT * i;
-            if (idx < total) buf[idx] = r[i];
-        }
-        __syncthreads();
-        if (panel + G < npanels) load(panel + G, tid);  // in flight during the passes
+        load(panel + G, tid);  // in flight during the passes (all masked past the last panel)
         DifSeq<T, INV, Rs...>::run(buf, L, 1, W, 1, W, tw, tid, NT);
         const int j0 = panel * W;
         const int w = min(W, N - j0);
@@ -170,14 +186,16 @@ __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ da
         for (int i = 0; i < MAXE; ++i) {
             const int idx = tid + NT * i;
             const int k = idx / W, v = idx % W;
-            if (idx < total && v < w) {
-                int src = k + sh;
-                if (src >= L) src -= L;
-                const cpx<T> y = buf[perm[src] * W + v];
-                data[(int64_t)k * ld + j0 + v] = INV ? scl(y, scale) : y;
-            }
+            const bool ok = idx < total && v < w;
+            int src = k + sh;
+            if (src >= L) src -= L;
+            const cpx<T> y = buf[ok ? perm[src] * W + v : 0];
+            const uint32_t off = ok ? (uint32_t)(((int64_t)k * ld + j0 + v) * (int64_t)sizeof(cpx<T>)) : kBufSkip;
+            buf_store(INV ? scl(y, scale) : y, rsrc, off);
         }
-        __syncthreads();  // the panel's LDS is rewritten next
+        __syncthreads();  // every lane has read its outputs from LDS
+        fill(tid);
+        __syncthreads();
     }
 }
 
@@ -195,6 +213,11 @@ template <typename T, bool INV, int W, int... Rs>
 static void launch_az_pipe(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
     constexpr int NT = 512;
     constexpr int L = (Rs * ...);
+    if (((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>) >= ((int64_t)1 << 31)) {
+        // 32-bit buffer offsets: larger arrays take the one-panel kernel
+        launch_az_fixed<T, INV, W, Rs...>(s, fl, data, ld, N, scale, sh);
+        return;
+    }
     constexpr int MAXE = (L * W + NT - 1) / NT;
     const int npanels = (N + W - 1) / W;
     const size_t lds = (size_t)L * W * sizeof(cpx<T>) + (size_t)L * (sizeof(cpx<T>) + sizeof(int32_t));
@@ -355,7 +378,7 @@ int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, in
 }
 
 // ----------------------------------------------------------- column table
-__global__ void az_columns_kernel(s1g_azimuth_desc d, AzColumn* __restrict__ cols) {
+__global__ void az_columns_kernel(s1g_azimuth_desc d, AzColumn* __restrict__ cols, float* __restrict__ amp32) {
 #pragma clang fp contract(off)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d.n_samples) return;
@@ -375,11 +398,57 @@ __global__ void az_columns_kernel(s1g_azimuth_desc d, AzColumn* __restrict__ col
     c.ct = c.c * inv2pi;
     c.btf = c.bt - rint(c.bt);
     cols[j] = c;
+    amp32[j] = (float)c.amp;
+}
+
+// Tolerances of the linear-geometry test: the migration R_j / dR to 1e-4
+// samples (a shift error below 1e-4 u / 2), the phase coefficient bt to
+// 5e-8 turns and t_corr to 5e-8 turns over the Doppler band; the FP32
+// chain's phases are good to about 1e-7 rad besides.
+constexpr double kLinTolRdr = 1e-4;
+constexpr double kLinTolTurns = 5e-8;
+
+// One workgroup: whether the columns share v_rel and f_dc and have R_j / dR,
+// bt and t_corr linear in j (fit through the first and last column).
+__global__ __launch_bounds__(1024) void az_uniform_kernel(const AzColumn* __restrict__ cols, int N, double prf,
+                                                          AzUniform* __restrict__ out) {
+    __shared__ int s_bad;
+    const int tid = threadIdx.x;
+    const AzColumn c0 = cols[0], c1 = cols[N - 1];
+    const double den = N > 1 ? (double)(N - 1) : 1.0;
+    const double drdr = (c1.rdr - c0.rdr) / den, dbt = (c1.bt - c0.bt) / den, dct = (c1.ct - c0.ct) / den;
+    const double tol_ct = kLinTolTurns / (0.5 * fabs(prf) + fabs(c0.fdc) + 1.0);
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    int bad = 0;
+    for (int j = tid; j < N; j += blockDim.x) {
+        const AzColumn c = cols[j];
+        bad |= c.a != c0.a || c.fdc != c0.fdc || !(fabs(c.rdr - (c0.rdr + j * drdr)) <= kLinTolRdr) ||
+               !(fabs(c.bt - (c0.bt + j * dbt)) <= kLinTolTurns) || !(fabs(c.ct - (c0.ct + j * dct)) <= tol_ct);
+    }
+    if (bad) atomicOr(&s_bad, 1);
+    __syncthreads();
+    if (tid == 0) {
+        AzUniform u;
+        u.ok = s_bad ? 0 : 1;
+        u.a = c0.a;
+        u.fdc = c0.fdc;
+        u.rdr0 = c0.rdr;
+        u.drdr = drdr;
+        u.bt0 = c0.bt;
+        u.dbt = dbt;
+        u.ct0 = c0.ct;
+        u.dct = dct;
+        *out = u;
+    }
 }
 
 int launch_az_columns(hipStream_t s, const s1g_azimuth_desc& d, AzColumn* cols) {
-    dim3 b(256), g((d.n_samples + 255) / 256);
-    hipLaunchKernelGGL(az_columns_kernel, g, b, 0, s, d, cols);
+    const int n = d.n_samples;
+    dim3 b(256), g((n + 255) / 256);
+    hipLaunchKernelGGL(az_columns_kernel, g, b, 0, s, d, cols, az_amp_of(cols, n));
+    hipLaunchKernelGGL(az_uniform_kernel, dim3(1), dim3(1024), 0, s, (const AzColumn*)cols, n, d.prf,
+                       az_uniform_of(cols, n));
     return (int)hipGetLastError();
 }
 
@@ -805,7 +874,183 @@ static void launch_rcmc32_run(hipStream_t s, const s1g_azimuth_desc& d, const Az
                        (cpx<float>*)out, ld_out, cap);
 }
 
-int rcmc_variants(int prec) { return prec == S1G_F32 ? 5 : 3; }
+// 1/D - 1 and 1 - D (D = sqrt(1 - u)): the series to u^5 up to kSeriesU,
+// else the square root without cancellation.
+__device__ __forceinline__ void migration_terms(double u, double& idm1, double& omd) {
+    if (u <= kSeriesU) {
+        idm1 = u * (0.5 + u * (0.375 + u * (0.3125 + u * (0.2734375 + u * 0.24609375))));
+        omd = u * (0.5 + u * (0.125 + u * (0.0625 + u * (0.0390625 + u * 0.02734375))));
+    } else {
+        const double D = sqrt(1.0 - u);
+        omd = u / (1.0 + D);
+        idm1 = omd / D;
+    }
+}
+
+// Tile slack for the spread of the integer migration over one workgroup
+// (at most a few samples at Sentinel-1 geometries).
+constexpr int kLinSlack = 64;
+
+// FP32 chain, linear column geometry (AzUniform::ok).  With v_rel and f_dc
+// the same in every column, D, 1/D - 1 and 1 - D depend on the Doppler row
+// only; R_j / dR, bt and t_corr are linear in j, so the migration is
+// shift_j = (rdr0 + j drdr)(1/D - 1) and the phase in turns
+// pt_j = P0 + j P1 with P0 = bt0 D + ct0 f, P1 = dbt D + dct f (FP64, the
+// fraction taken before the FP32 sin/cos).  A thread takes EPT consecutive
+// samples: the migration moves by (1/D - 1) drdr (below 6e-6 samples here)
+// per sample, so when the run keeps one integer shift and its offset spread
+// is within kInterpSpan the windowed-sinc weights are evaluated at its first
+// and last sample and interpolated linearly for the others (the offset is
+// linear in the sample), and the run reads its 2*8 + EPT taps from the LDS
+// row tile once.  Other runs, and tiles that would exceed kLinSlack, take
+// exact per-sample weights.  Without linear geometry every sample takes the
+// column table's geometry and exact weights.
+template <int NT, int EPT>
+__global__ __launch_bounds__(NT) void rcmc32_lin_kernel(s1g_azimuth_desc d, const AzColumn* __restrict__ cols,
+                                                        const AzUniform* __restrict__ uni,
+                                                        const float* __restrict__ amp32,
+                                                        const cpx<float>* __restrict__ in, int64_t ld_in,
+                                                        cpx<float>* __restrict__ out, int64_t ld_out) {
+    constexpr int PER = NT * EPT;
+    constexpr int CAP = PER + 2 * kTaps + kLinSlack;
+    constexpr int NW = 2 * kTaps + EPT;
+    __shared__ cpx<float> tile[CAP];
+    const int N = d.n_samples, L = d.n_lines;
+    const int i = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int j0 = blockIdx.x * PER;
+    const bool rcmc = d.apply_rcmc != 0;
+    double fa;
+    {
+#pragma clang fp contract(off)
+        const double start = -d.prf / 2, end = d.prf / 2;
+        const double step = (end - start) / (L - 1);
+        fa = start + i * step;
+    }
+    const cpx<float>* row = in + (int64_t)i * ld_in;
+    cpx<float>* orow = out + (int64_t)i * ld_out;
+    const int js = j0 + tid * EPT;
+    cpx<float> acc[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) acc[e] = mk(0.0f, 0.0f);
+    // per-sample taps and exact weights (bounds-checked global reads)
+    auto sample = [&](int e, int n0, float fr) {
+        float w[2 * kTaps + 1];
+        sinc_weights(fr, w);
+#pragma unroll
+        for (int k = 0; k <= 2 * kTaps; ++k) {
+            const int idx = n0 - kTaps + k;
+            if (idx < 0 || idx >= N) continue;
+            const cpx<float> x = row[idx];
+            acc[e].x += x.x * w[k];
+            acc[e].y += x.y * w[k];
+        }
+    };
+    if (uni->ok) {
+        const AzUniform g = *uni;
+        const double f = fa + g.fdc;
+        double idm1, omd;
+        migration_terms(g.a * (f * f), idm1, omd);
+        if (!rcmc) idm1 = 0.0;
+        const double D = 1.0 - omd;
+        const double P0 = g.bt0 * D + g.ct0 * f;
+        const double P1 = g.dbt * D + g.dct * f;
+        // the workgroup's tile: every tap its runs can reach
+        const int jl = min(j0 + PER, N) - 1;
+        const double shA = (g.rdr0 + j0 * g.drdr) * idm1, shB = (g.rdr0 + jl * g.drdr) * idm1;
+        const int tlo = j0 + (int)floor(fmin(shA, shB)) - kTaps;
+        const int thi = jl + (int)floor(fmax(shA, shB)) + kTaps;
+        const bool use_tile = rcmc && thi - tlo + 1 <= CAP;
+        if (use_tile)
+            for (int k = tlo + tid; k <= thi; k += NT) tile[k - tlo] = (k >= 0 && k < N) ? row[k] : mk(0.0f, 0.0f);
+        __syncthreads();
+        if (js < N) {
+            const double s0 = (g.rdr0 + js * g.drdr) * idm1;
+            const double s1 = (g.rdr0 + (js + EPT - 1) * g.drdr) * idm1;
+            const double f0 = floor(s0), f1 = floor(s1);
+            if (!rcmc) {
+#pragma unroll
+                for (int e = 0; e < EPT; ++e)
+                    if (js + e < N) acc[e] = row[js + e];
+            } else if (use_tile && f0 == f1 && js + EPT <= N && fabs(s1 - s0) <= kInterpSpan) {
+                float w0[2 * kTaps + 1], dw[2 * kTaps + 1];
+                sinc_weights((float)(s0 - f0), w0);
+                sinc_weights((float)(s1 - f0), dw);
+#pragma unroll
+                for (int k = 0; k <= 2 * kTaps; ++k) dw[k] -= w0[k];
+                const cpx<float>* tp = tile + (js + (int)f0 - kTaps - tlo);
+                cpx<float> x[NW];
+#pragma unroll
+                for (int m = 0; m < NW; ++m) x[m] = tp[m];
+#pragma unroll
+                for (int e = 0; e < EPT; ++e) {
+                    const float t = (float)e / (float)(EPT - 1);
+#pragma unroll
+                    for (int k = 0; k <= 2 * kTaps; ++k) {
+                        const float w = fmaf(t, dw[k], w0[k]);
+                        acc[e].x += x[e + k].x * w;
+                        acc[e].y += x[e + k].y * w;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < EPT; ++e) {
+                    const int j = js + e;
+                    if (j >= N) continue;
+                    const double sh = (g.rdr0 + j * g.drdr) * idm1;
+                    const double fl = floor(sh);
+                    sample(e, j + (int)fl, (float)(sh - fl));
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                const int j = js + e;
+                if (j >= N) continue;
+                const double pt = fma((double)j, P1, P0);
+                const float pr = (float)(pt - rint(pt));
+                const cpx<float> ph = mk(__builtin_amdgcn_cosf(pr), __builtin_amdgcn_sinf(pr));
+                acc[e] = cmul(scl(acc[e], amp32[j]), ph);
+            }
+        }
+        // hand the runs' samples back through LDS so that consecutive lanes
+        // store consecutive samples (whole lines, no partial-line writes)
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) tile[tid * EPT + e] = acc[e];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int q = e * NT + tid;
+            if (j0 + q < N) orow[j0 + q] = tile[q];
+        }
+    } else if (js < N) {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int j = js + e;
+            if (j >= N) continue;
+            int n0;
+            double fr;
+            float c, sn, a;
+            geo32_series(cols[j], fa, j, rcmc, n0, fr, c, sn, a);
+            if (rcmc) sample(e, n0, (float)fr);
+            else acc[e] = row[j];
+            orow[j] = cmul(scl(acc[e], a), mk(c, sn));
+        }
+    }
+}
+
+template <int NT, int EPT>
+static void launch_rcmc32_lin(hipStream_t s, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
+                              int64_t ld_in, void* out, int64_t ld_out) {
+    const int per = NT * EPT;
+    dim3 grid((unsigned)((d.n_samples + per - 1) / per), (unsigned)d.n_lines);
+    AzColumn* c = const_cast<AzColumn*>(cols);
+    hipLaunchKernelGGL((rcmc32_lin_kernel<NT, EPT>), grid, dim3(NT), 0, s, d, cols,
+                       (const AzUniform*)az_uniform_of(c, d.n_samples), (const float*)az_amp_of(c, d.n_samples),
+                       (const cpx<float>*)in, ld_in, (cpx<float>*)out, ld_out);
+}
+
+int rcmc_variants(int prec) { return prec == S1G_F32 ? 7 : 3; }
 
 template <typename T, int NT, int EPT>
 static void launch_rcmc_t(hipStream_t s, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
@@ -819,11 +1064,14 @@ static void launch_rcmc_t(hipStream_t s, const s1g_azimuth_desc& d, const AzColu
 
 // Variants (threads x samples per thread): 0 = 256 x 8, 1 = 256 x 4, 2 = 512 x 4;
 // FP32 also runs of 9 consecutive samples per thread (rcmc32_run_kernel):
-// 3 = 256 threads, 4 = 128 threads.
+// 3 = 256 threads, 4 = 128 threads; and the linear-geometry kernel
+// (rcmc32_lin_kernel), 256 threads with runs of 5 = 16, 6 = 8 samples.
 int launch_rcmc_mf(hipStream_t s, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
                    int64_t ld_in, void* out, int64_t ld_out, int variant) {
     if (prec == S1G_F32) {
-        if (variant == 3) launch_rcmc32_run<256, 9>(s, d, cols, in, ld_in, out, ld_out);
+        if (variant == 5) launch_rcmc32_lin<256, 16>(s, d, cols, in, ld_in, out, ld_out);
+        else if (variant == 6) launch_rcmc32_lin<256, 8>(s, d, cols, in, ld_in, out, ld_out);
+        else if (variant == 3) launch_rcmc32_run<256, 9>(s, d, cols, in, ld_in, out, ld_out);
         else if (variant == 4) launch_rcmc32_run<128, 9>(s, d, cols, in, ld_in, out, ld_out);
         else if (variant == 1) launch_rcmc_t<float, 256, 4>(s, d, cols, in, ld_in, out, ld_out);
         else if (variant == 2) launch_rcmc_t<float, 512, 4>(s, d, cols, in, ld_in, out, ld_out);

@@ -396,7 +396,7 @@ int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int
             int64_t ld_out) {
     if (d.n_lines <= 1 || d.n_samples <= 0) return fail(c, S1G_EARG, "azimuth stage needs >= 2 lines%s");
     if (!d.slant_ranges || !d.v_rel || !d.f_dc || !d.t_corr) return fail(c, S1G_EARG, "null geometry array%s");
-    int rc = ensure(c, (void**)&c->cols, &c->cols_n, sizeof(AzColumn) * (size_t)d.n_samples);
+    int rc = ensure(c, (void**)&c->cols, &c->cols_n, az_table_bytes(d.n_samples));
     if (rc) return rc;
     rc = launch_az_columns(c->stream, d, c->cols);
     if (rc) return hip_fail(c, (hipError_t)rc, "azimuth columns");
@@ -936,7 +936,7 @@ static int prepare_burst(s1g_ctx* c, const s1g_burst_desc& b, int prec, size_t* 
         }
     }
     *ws_need = std::max(*ws_need, esize(prec) * (size_t)L * N);
-    *cols_need = std::max(*cols_need, sizeof(AzColumn) * (size_t)N);
+    *cols_need = std::max(*cols_need, az_table_bytes(N));
     return rc;
 }
 

@@ -382,7 +382,10 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     extern __shared__ __align__(16) uint32_t smem[];  // base 16-aligned after the static words
     __shared__ int s_st, s_st_qo;  // status after IE..QE; status of the QO walk
     const int p = blockIdx.x;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // wave index as a scalar: the block walk under `wave == 0` is then
+    // uniform control flow rather than exec-masked
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (p >= n_packets) return;
 
     uint32_t* staged = smem;                                     // lds_words

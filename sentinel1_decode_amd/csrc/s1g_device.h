@@ -39,6 +39,48 @@ __device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
 template <typename T>
 __device__ __forceinline__ cpx<T> scl(cpx<T> a, T s) { return mk(a.x * s, a.y * s); }
 
+// ------------------------------------------------------- buffer accesses
+// Raw buffer loads and stores through a resource of `bytes` bytes at
+// `base`: an offset at or beyond the range reads zero and drops the store,
+// so a masked element needs no branch and every wave issues the same
+// number of memory instructions (the waits on them can then be counted).
+// Offsets are 32-bit: ranges below 2 GiB, with kBufSkip as the offset of a
+// masked element.
+constexpr uint32_t kBufSkip = 0x80000000u;
+constexpr int kBufConfig = 0x00020000;  // raw buffer, 32-bit data format (gfx9 family)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufConfig);
+}
+typedef int s1g_v2i __attribute__((ext_vector_type(2)));
+typedef int s1g_v4i __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ cpx<T> buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ cpx<float> buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const s1g_v2i v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+    return mk(__int_as_float(v.x), __int_as_float(v.y));
+}
+template <>
+__device__ __forceinline__ cpx<double> buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const s1g_v4i v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return mk(__hiloint2double(v.y, v.x), __hiloint2double(v.w, v.z));
+}
+__device__ __forceinline__ void buf_store(cpx<float> a, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    s1g_v2i v;
+    v.x = __float_as_int(a.x);
+    v.y = __float_as_int(a.y);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(cpx<double> a, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    s1g_v4i v;
+    v.x = __double2loint(a.x);
+    v.y = __double2hiint(a.x);
+    v.z = __double2loint(a.y);
+    v.w = __double2hiint(a.y);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
+}
+
 // ------------------------------------------------- compile-time trigonometry
 // cos(2 pi num / den) and sin(2 pi num / den) for integers, evaluated at
 // compile time: exact octant reduction on the rational angle, then Taylor

@@ -41,6 +41,27 @@ struct AzColumn {
     double btf;   // bt - rint(bt)
 };
 
+// Summary of the column table for the FP32 chain's linear-geometry kernel:
+// `ok` when v_rel and f_dc are the same in every column and R_j / dR, the
+// phase coefficient bt and t_corr are linear in j (to the tolerances of
+// az_uniform_kernel), with the fit's base values and slopes.
+struct AzUniform {
+    int ok;
+    double a, fdc;        // lambda^2 / (4 v^2), Doppler centroid
+    double rdr0, drdr;    // R_j / dR = rdr0 + j drdr
+    double bt0, dbt;      // bt_j = bt0 + j dbt (turns)
+    double ct0, dct;      // t_corr_j = ct0 + j dct
+};
+
+// Bytes of the column table of n columns: AzColumn[n], AzUniform, float amplitude[n].
+inline size_t az_table_bytes(int n) {
+    return ((sizeof(AzColumn) * (size_t)n + sizeof(AzUniform) + 15) & ~(size_t)15) + sizeof(float) * (size_t)n;
+}
+inline AzUniform* az_uniform_of(AzColumn* cols, int n) { return (AzUniform*)(cols + n); }
+inline float* az_amp_of(AzColumn* cols, int n) {
+    return (float*)((char*)cols + ((sizeof(AzColumn) * (size_t)n + sizeof(AzUniform) + 15) & ~(size_t)15));
+}
+
 int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
                         const int32_t* nqs, int n_packets, int max_bytes, int max_quads, void* out, int64_t ld_out,
                         int32_t* status);

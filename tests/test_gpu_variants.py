@@ -108,7 +108,7 @@ def test_rcmc_variants(vctx, rd_case, prec, tol):
     dt = torch.complex128 if prec == F64 else torch.complex64
     x = torch.from_numpy(ch["rd"]).to(dt).cuda()
     n = _variant_count(vctx, "rcmc_mf", lambda: vctx.rcmc_matched_filter(x, db.azimuth_desc()))
-    assert n == (5 if prec == F32 else 3)
+    assert n == (7 if prec == F32 else 3)
     for v in range(n):
         vctx.force_variant("rcmc_mf", v)
         y = vctx.rcmc_matched_filter(x, db.azimuth_desc())
@@ -119,7 +119,8 @@ def test_rcmc_variants(vctx, rd_case, prec, tol):
 
 @pytest.mark.parametrize("lam_scale", [1.0, 4.0, 12.0, 60.0])
 def test_rcmc_runs_against_fp64(vctx, rd_case, lam_scale):
-    """FP32 run kernels (variants 3, 4) against the FP64 path on the same input.
+    """FP32 run kernels (variants 3, 4; linear geometry 5, 6) against the FP64
+    path on the same input.
     A longer wavelength raises u = lambda^2 f^2 / (4 v^2) (about 1e-5 at
     the band edge here): x4 spreads the fractional offsets of the outer
     rows' runs past the interpolation limit (per-sample weights), x12 and
@@ -133,13 +134,45 @@ def test_rcmc_runs_against_fp64(vctx, rd_case, lam_scale):
     vctx.force_variant("rcmc_mf", 0)
     ref = vctx.rcmc_matched_filter(x64, desc).cpu().numpy()
     x32 = x64.to(torch.complex64)
-    for v in (0, 3, 4):
+    for v in (0, 3, 4, 5, 6):
         vctx.force_variant("rcmc_mf", v)
         y = vctx.rcmc_matched_filter(x32, desc)
         torch.cuda.synchronize()
-        assert vctx.last_variant("rcmc_mf") == (v, 5)
+        assert vctx.last_variant("rcmc_mf") == (v, 7)
         assert rel_l2(y.cpu().numpy(), ref) < 2e-6, (lam_scale, v)
     vctx.force_variant("rcmc_mf", -1)
+
+
+@pytest.mark.parametrize("field", ["v_rel", "f_dc", "slant_ranges"])
+def test_rcmc_linear_kernel_general_geometry(vctx, rd_case, field):
+    """The linear-geometry kernels (variants 5, 6) on columns that break
+    their assumptions (v_rel or f_dc varying with range, slant ranges off a
+    straight line): the per-sample path against the FP64 path."""
+    burst, ch = rd_case
+    db = pipeline.from_synth(burst)
+    desc = db.azimuth_desc()
+    n = db.n_samples
+    ramp = torch.linspace(0.0, 1.0, n, dtype=torch.float64, device="cuda")
+    if field == "v_rel":
+        arr = db.d_v + 25.0 * ramp
+        desc.v_rel = arr.data_ptr()
+    elif field == "f_dc":
+        arr = db.d_fdc + 40.0 * ramp
+        desc.f_dc = arr.data_ptr()
+    else:
+        arr = db.d_r + 3.0 * ramp * ramp
+        desc.slant_ranges = arr.data_ptr()
+    x64 = torch.from_numpy(ch["rd"]).cuda()
+    vctx.force_variant("rcmc_mf", 0)
+    ref = vctx.rcmc_matched_filter(x64, desc).cpu().numpy()
+    x32 = x64.to(torch.complex64)
+    for v in (5, 6):
+        vctx.force_variant("rcmc_mf", v)
+        y = vctx.rcmc_matched_filter(x32, desc)
+        torch.cuda.synchronize()
+        assert rel_l2(y.cpu().numpy(), ref) < 2e-6, (field, v)
+    vctx.force_variant("rcmc_mf", -1)
+    del arr
 
 
 def test_autotuned_chain_and_forced_graph(vctx):
