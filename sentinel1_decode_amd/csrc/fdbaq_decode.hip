@@ -347,6 +347,12 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
     const int q0 = 128 * b;
     const int cnt = (b == nb - 1) ? nq - q0 : 128;
     T* dst = orow + 2 * (2 * (int64_t)q0 + (ch & 1)) + (ch >> 1);
+    // Software pipeline: the reconstruction level of symbol s is read from
+    // LDS while symbol s + 1 is decoded, and symbol s is stored after that,
+    // so the LDS latency overlaps the next symbol's bit work.
+    double lvl = 0.0;
+    int m_prev = 0;
+    uint32_t neg_prev = 0;
     for (int s = 0; s < cnt; ++s) {
         if (valid < 32) {
             cur |= (uint64_t)nxt << (32 - valid);
@@ -359,10 +365,20 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
         const int clo = fdbaq_clo(code10);
         const int len = fdbaq_sym_len(code10, clo, tab, x4);
         const int m = fdbaq_mcode_tab(code10, clo, mbase, mshift, mmask, x4);
-        const double mg = simple ? (m < flag ? (double)m : sv) : __dmul_rn(nrow[m], sf);
+        const double lvl_next = nrow[m];
         cur <<= len;
         valid -= len;
-        dst[4 * (int64_t)s] = (T)((v >> 10) ? -mg : mg);
+        if (s > 0) {
+            const double mg = simple ? (m_prev < flag ? (double)m_prev : sv) : __dmul_rn(lvl, sf);
+            dst[4 * (int64_t)(s - 1)] = (T)(neg_prev ? -mg : mg);
+        }
+        lvl = lvl_next;
+        m_prev = m;
+        neg_prev = v >> 10;
+    }
+    if (cnt > 0) {
+        const double mg = simple ? (m_prev < flag ? (double)m_prev : sv) : __dmul_rn(lvl, sf);
+        dst[4 * (int64_t)(cnt - 1)] = (T)(neg_prev ? -mg : mg);
     }
 }
 
@@ -392,7 +408,11 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     int32_t* bstart = (int32_t*)(smem + lds_words);              // 4 * max_blocks
     uint8_t* bthr = (uint8_t*)(bstart + 4 * max_blocks);         // max_blocks
     uint8_t* bbrc = bthr + max_blocks;                           // max_blocks
-    double* nrl = (double*)(((uintptr_t)(bbrc + max_blocks) + 7) & ~(uintptr_t)7);  // 5 * 16
+    // 5 * 16 levels, 8-byte aligned; offset in words from the LDS base (a
+    // cast through an integer would turn the table reads into FLAT loads,
+    // which wait on every outstanding global store)
+    const int nrl_words = (lds_words + 4 * max_blocks + (2 * max_blocks + 3) / 4 + 1) & ~1;
+    double* nrl = (double*)(smem + nrl_words);
     for (int i = tid; i < 80; i += 128) nrl[i] = c_nrl[i >> 4][i & 15];
 
     const int nq = nqs[p];
@@ -408,11 +428,12 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     PacketWords<ALL> words;
     {
         const uint8_t* a = ud + offs[p];
-        const uintptr_t au = (uintptr_t)a;
+        const int off = (int)((uintptr_t)a & 3);
         words.lds = staged;
         words.cap = lds_words;
-        words.al = (const uint32_t*)(au & ~(uintptr_t)3);
-        words.off = (int)(au & 3);
+        // aligned by pointer arithmetic, so the reads stay global loads
+        words.al = (const uint32_t*)(a - off);
+        words.off = off;
         words.last_al = (words.off + nbytes - 1) >> 2;
         words.nbytes = nbytes;
         words.wlim = (nbytes + 3) / 4 + kPadWords;
@@ -516,10 +537,10 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     const int max_words = (max_bytes + 3) / 4;
     const int lds_words = std::min(max_words + kPadWords, kLdsWords);
     const int max_blocks = (max_quads + 127) / 128 + 1;
-    size_t lds = sizeof(uint32_t) * (size_t)lds_words + sizeof(int32_t) * 4 * (size_t)max_blocks +
-                 2 * (size_t)max_blocks;
-    lds = (lds + 7) & ~(size_t)7;
-    lds += sizeof(double) * 80;
+    // staged words, block starts, THIDX and BRC bytes, then the 80 levels at
+    // the same word offset the kernel computes
+    const int nrl_words = (lds_words + 4 * max_blocks + (2 * max_blocks + 3) / 4 + 1) & ~1;
+    const size_t lds = sizeof(uint32_t) * (size_t)nrl_words + sizeof(double) * 80;
     if (lds > 160 * 1024) return -1;
     dim3 grid(n_packets), block(128);
     const bool all = max_words + kPadWords <= kLdsWords;
