@@ -157,29 +157,33 @@ __device__ __forceinline__ uint64_t win64(const Win5& w, int o) {
 
 __device__ __forceinline__ uint64_t len_table(int brc) { return fdbaq_len_table(brc); }
 
-// Length of the symbol at the top of `hi` (sign bit, then the code).
-// BRC 0-3: the code length is med3(clo + 1, lo, hi) with (lo, hi) = (1, 3),
-// (1, 4), (1, 6), (2, 8), so the symbol length is med3(clo + 2, lo + 1,
-// hi + 1) (lo1, hi1 below).  BRC 4 reads its nibble table `tab` and adds
-// the second code bit for its '0x' codes.
-template <bool B4>
-__device__ __forceinline__ int sym_len_hi(uint32_t hi, uint64_t tab, int lo1, int hi1) {
-    const int clo = (int)__clz(~(hi << 1));
-    if constexpr (!B4) {
-        return min(max(clo + 2, lo1), hi1);
+// Length of the symbol whose code bits head `x`, for code BRC: the walks
+// keep their 64-bit window complemented and already past the sign bit, so
+// the count of leading ones of the code is a leading-zero count of x (32
+// when x is 0).  BRC 0-3: the code length is med3(clo + 1, lo, hi) with
+// (lo, hi) = (1, 3), (1, 4), (1, 6), (2, 8), so the symbol length is
+// 2 + med3(clo, lo - 1, hi - 1).  BRC 4 reads its nibble table by
+// min(clo, 10) and adds the second code bit for its '0x' codes.
+template <int BRC>
+__device__ __forceinline__ int sym_len_n(uint32_t x) {
+    const uint32_t clo = x ? (uint32_t)__builtin_clz(x) : 32u;
+    if constexpr (BRC < 3) {
+        constexpr uint32_t hi = BRC == 0 ? 2 : BRC == 1 ? 3 : 5;
+        return 2 + (int)min(clo, hi);
+    } else if constexpr (BRC == 3) {
+        return 2 + (int)min(max(clo, 1u), 7u);
     } else {
-        const int c = min(clo, 10);
-        const int len = (int)((tab >> (4 * c)) & 15u);
-        const uint32_t extra = c == 0 ? ((hi >> 29) & 1u) : 0u;
+        const int c = (int)min(clo, 10u);
+        const int len = (int)((kLen4 >> (4 * c)) & 15u);
+        const uint32_t extra = c == 0 ? ((~x >> 30) & 1u) : 0u;
         return 1 + len + (int)extra;
     }
 }
 
-// (lo + 1, hi + 1) of the BRC 0-3 code-length rule.
-__device__ __forceinline__ void len_bounds(int brc, int& lo1, int& hi1) {
-    lo1 = brc == 3 ? 3 : 2;
-    hi1 = brc == 0 ? 4 : brc == 1 ? 5 : brc == 2 ? 7 : 9;
-}
+// The walk window: the 64 bits from a symbol's sign bit, complemented and
+// shifted past that sign bit.
+__device__ __forceinline__ uint64_t walk_window(uint64_t bits) { return ~(bits << 1); }
+
 
 // Value of the lane to the left (lane 0 gets 0): DPP wave_shr:1.
 __device__ __forceinline__ int from_left(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
@@ -216,9 +220,10 @@ __device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
 // *hdr receives the 8 bits that follow the end (the next block's BRC or
 // THIDX field).  Reads stay below word `wlim`; once `start` passes `nbits`
 // the block cannot end inside the packet and the walk stops there.
-template <bool B4, bool ALL>
-__device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, uint64_t tab, int lo1,
-                              int hi1, int nsym, int delta, int lane, uint32_t* hdr) {
+template <int BRC, bool ALL>
+__device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, int nsym, int delta, int lane,
+                          uint32_t* hdr) {
+    constexpr bool B4 = BRC == 4;
     for (;;) {
         if (start > nbits) {
             *hdr = 0;
@@ -238,18 +243,18 @@ __device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, u
         w.w4 = words(w0 + 4);
         // window offset of slice offset o is o + zero (zero <= 31 + kOverlap)
         const int zero = (int)(org & 31) + (int)(s - org);
-        uint64_t win = win64(w, (int)(org & 31));
+        uint64_t win = walk_window(win64(w, (int)(org & 31)));
         uint32_t mask = 0;
         int o = (int)(org - s);
         // lead-in: the overlap before the slice (nothing recorded)
         while (o < 0) {
-            const int l = sym_len_hi<B4>((uint32_t)(win >> 32), tab, lo1, hi1);
+            const int l = sym_len_n<BRC>((uint32_t)(win >> 32));
             o += l;
             win <<= l;
         }
         while (o < delta) {
             mask |= 1u << o;
-            const int l = sym_len_hi<B4>((uint32_t)(win >> 32), tab, lo1, hi1);
+            const int l = sym_len_n<BRC>((uint32_t)(win >> 32));
             o += l;
             win <<= l;
         }
@@ -269,10 +274,10 @@ __device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, u
                 if (!((mask >> entry) & 1u)) {
                     mask = 0;
                     o = entry;
-                    uint64_t wr = win64(w, zero + entry);
+                    uint64_t wr = walk_window(win64(w, zero + entry));
                     while (o < delta) {
                         mask |= 1u << o;
-                        const int l = sym_len_hi<B4>((uint32_t)(wr >> 32), tab, lo1, hi1);
+                        const int l = sym_len_n<BRC>((uint32_t)(wr >> 32));
                         o += l;
                         wr <<= l;
                     }
@@ -490,11 +495,14 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
             // slice width from the previous block of this channel (or a BRC guess)
             const int est = b ? prev_len : nsym * (brc + 3);
             const int delta = min(max((est + 63) / 64 + 1, 2), 32);
-            int lo1, hi1;
-            len_bounds(brc, lo1, hi1);
-            const int end =
-                brc == 4 ? walk_block<true, ALL>(words, nbits, pos, kLen4, lo1, hi1, nsym, delta, lane, &hdr)
-                         : walk_block<false, ALL>(words, nbits, pos, 0, lo1, hi1, nsym, delta, lane, &hdr);
+            int end;
+            switch (brc) {
+                case 0: end = walk_block<0, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
+                case 1: end = walk_block<1, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
+                case 2: end = walk_block<2, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
+                case 3: end = walk_block<3, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
+                default: end = walk_block<4, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
+            }
             prev_len = (int)(end - pos);
             pos = end;
             if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
