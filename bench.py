@@ -108,6 +108,30 @@ def run_gpu(burst, steps, warmup):
     return res, L, N
 
 
+# FP32 kernel of each stage variant (the autotuner's numbering in the
+# library: range = overlap-save block 16384 or 8192; azimuth FFT = 64-byte
+# panels, 32-byte panels, generic, persistent 64-byte panels; RCMC = three
+# per-sample tilings, two run kernels, two linear-geometry kernels).
+_STAGE_KERNELS = {
+    "decode": ["fdbaq_decode_kernel<float"],
+    "range": ["range_os3_kernel<float, 16384", "range_os3_kernel<float, 8192"],
+    "az_fft": ["az_fft_fixed_kernel<float, false, 512, 8,", "az_fft_fixed_kernel<float, false, 512, 4,",
+               "az_fft_generic_kernel<float, false", "az_fft_pipe_kernel<float, false"],
+    "az_ifft": ["az_fft_fixed_kernel<float, true, 512, 8,", "az_fft_fixed_kernel<float, true, 512, 4,",
+                "az_fft_generic_kernel<float, true", "az_fft_pipe_kernel<float, true"],
+    "rcmc_mf": ["rcmc_mf_kernel<float, 256, 8>", "rcmc_mf_kernel<float, 256, 4>", "rcmc_mf_kernel<float, 512, 4>",
+                "rcmc32_run_kernel<256, 9>", "rcmc32_run_kernel<128, 9>", "rcmc32_lin_kernel<256, 16>",
+                "rcmc32_lin_kernel<256, 8>"],
+}
+
+
+def kernel_name(stage, variant):
+    names = _STAGE_KERNELS.get(stage, [])
+    if stage == "decode":
+        return names[0]
+    return names[variant] if 0 <= variant < len(names) else "?"
+
+
 def measured_traffic(kernel_prefix, L, N):
     """HBM bytes per launch of a kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
@@ -388,10 +412,7 @@ def main():
     achieved = sb[dom] / (fp32["stages_ms"][dom] * 1e-3) / 1e9
     fft_frac = {s: stage_bytes(4, L, N, b_pkt)[s] / (fp32["stages_ms"][s] * 1e-3) / 1e9 / HBM_PEAK_GBS
                 for s in ("range", "az_fft", "az_ifft") if s in fp32["stages_ms"]}
-    kernel_of = {"decode": "fdbaq_decode_kernel<float>", "range": "range_os3_kernel<float",
-                 "az_fft": "az_fft_fixed_kernel<float, false", "rcmc_mf": "rcmc_mf_kernel<float",
-                 "az_ifft": "az_fft_fixed_kernel<float, true"}
-    traffic = measured_traffic(kernel_of.get(dom, "?"), L, N)
+    traffic = measured_traffic(kernel_name(dom, fp32["variants"].get(dom, [0, 1])[0]), L, N)
     line = {
         "metric": METRIC,
         "value": round(fp32["msamples_s"], 3),

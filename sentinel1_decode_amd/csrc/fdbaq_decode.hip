@@ -77,8 +77,8 @@ constexpr int kMaxPacketBytes = 1 << 27;
 // usually fallen into step with the true symbol boundaries on arrival.
 // kOverlap + 32 (the widest slice) + 11 (the longest symbol) <= 64 keeps
 // every walk inside one 64-bit window.
-constexpr int kOverlap = 16;
-constexpr int kOverlapB4 = 28;
+constexpr int kOverlap = 12;
+constexpr int kOverlapB4 = 24;
 
 // Packets with more BAQ blocks than this are rejected (S1G_PKT_ERR_ARG):
 // the block BRCs are kept in 4 registers per lane.
@@ -203,6 +203,21 @@ __device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
     return x + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
 }
 
+// Position of the j-th (1-based, 1 <= j <= popc(w)) set bit of w: a binary
+// search on popcounts, with no branch.
+__device__ __forceinline__ int select_bit(uint32_t w, int j) {
+    int pos = 0;
+#pragma unroll
+    for (int half = 16; half >= 1; half >>= 1) {
+        const int c = __popc(w & ((1u << half) - 1u));
+        const bool up = j > c;
+        j -= up ? c : 0;
+        w = up ? w >> half : w;
+        pos += up ? half : 0;
+    }
+    return pos;
+}
+
 // Position after `nsym` symbols of one BRC starting at bit `start`, found by
 // all 64 lanes.  Lane l owns the slice [start + l*delta, start + (l+1)*delta)
 // of a region of 64*delta bits:
@@ -297,22 +312,14 @@ __device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, i
         if (total >= nsym) {
             const unsigned long long bal = __ballot(P >= nsym);
             const int ls = __ffsll((long long)bal) - 1;
-            int endrel = 0;
-            uint32_t h = 0;
-            if (lane == ls) {
-                const int k = nsym - (P - cnt);  // block symbols starting in this slice, 1..cnt
-                if (k < cnt) {
-                    uint32_t m = live;
-                    for (int i = 0; i < k; ++i) m &= m - 1u;
-                    endrel = entry + __builtin_ctz(m);
-                } else {
-                    endrel = ex;
-                }
-                h = (uint32_t)(win64(w, zero + endrel) >> 56);
-            }
-            endrel = __builtin_amdgcn_readlane(endrel, ls);
+            // every lane forms its candidate without branching; lane ls's is
+            // the block's: k block symbols start in its slice (1..cnt), so the
+            // block ends at its (k+1)-th start, or at its exit when k == cnt
+            const int k = nsym - (P - cnt);
+            const int endrel = (k >= 1 && k < cnt) ? entry + select_bit(live, k + 1) : ex;
+            const uint32_t h = (uint32_t)(win64(w, min(zero + endrel, 95)) >> 56);
             *hdr = (uint32_t)__builtin_amdgcn_readlane((int)h, ls);
-            return start + ls * delta + endrel;
+            return start + ls * delta + __builtin_amdgcn_readlane(endrel, ls);
         }
         nsym -= total;
         start += 63 * delta + __builtin_amdgcn_readlane(ex, 63);
