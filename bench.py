@@ -110,15 +110,18 @@ def run_gpu(burst, steps, warmup):
 
 # FP32 kernel of each stage variant (the autotuner's numbering in the
 # library: range = overlap-save block 16384 or 8192; azimuth FFT = 64-byte
-# panels, 32-byte panels, generic, persistent 64-byte panels; RCMC = three
+# panels, 32-byte panels, generic, persistent 64-byte panels, the same with
+# the first and last passes in registers; RCMC = three
 # per-sample tilings, two run kernels, two linear-geometry kernels).
 _STAGE_KERNELS = {
     "decode": ["fdbaq_decode_kernel<float"],
     "range": ["range_os3_kernel<float, 16384", "range_os3_kernel<float, 8192"],
     "az_fft": ["az_fft_fixed_kernel<float, false, 512, 8,", "az_fft_fixed_kernel<float, false, 512, 4,",
-               "az_fft_generic_kernel<float, false", "az_fft_pipe_kernel<float, false"],
+               "az_fft_generic_kernel<float, false", "az_fft_pipe_kernel<float, false",
+               "az_fft_reg_kernel<float, false"],
     "az_ifft": ["az_fft_fixed_kernel<float, true, 512, 8,", "az_fft_fixed_kernel<float, true, 512, 4,",
-                "az_fft_generic_kernel<float, true", "az_fft_pipe_kernel<float, true"],
+                "az_fft_generic_kernel<float, true", "az_fft_pipe_kernel<float, true",
+                "az_fft_reg_kernel<float, true"],
     "rcmc_mf": ["rcmc_mf_kernel<float, 256, 8>", "rcmc_mf_kernel<float, 256, 4>", "rcmc_mf_kernel<float, 512, 4>",
                 "rcmc32_run_kernel<256, 9>", "rcmc32_run_kernel<128, 9>", "rcmc32_lin_kernel<256, 16>",
                 "rcmc32_lin_kernel<256, 8>"],

@@ -199,6 +199,119 @@ __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ da
     }
 }
 
+// Persistent kernel with the first and last DIF passes in registers.  For a
+// radix sequence (DIT order) RA, MIDS..., RZ the DIF passes run RZ (stride
+// L / RZ), the MIDS, and RA (contiguous, no twiddles).  Here each thread
+// loads whole first-pass butterflies (elements j + (L / RZ) r of a column)
+// straight from global memory, transforms and twiddles them in registers
+// and writes them to LDS; the middle passes run in LDS; and the last pass
+// reads its RA contiguous elements from LDS and stores each output straight
+// to its final row: buffer position p holds frequency ipos[p] (the inverse
+// of the digit-reversal map), which lands on row (ipos[p] - sh) mod L (the
+// fftshift, forward) or is scaled by 1/L (inverse).  That is two LDS passes
+// and two barriers per panel fewer than az_fft_pipe_kernel.  Prefetch,
+// XCD-aware dealing and the counted waits (buffer accesses, masked by
+// out-of-range offsets) are as there; the array must span less than 2 GiB.
+template <typename T, bool INV, int NT, int W, int RA, int RZ, int... MIDS>
+__global__ __launch_bounds__(NT) void az_fft_reg_kernel(cpx<T>* __restrict__ data, int64_t ld, int N, int npanels,
+                                                        const cpx<T>* __restrict__ tw_g,
+                                                        const int32_t* __restrict__ perm_g, T scale, int sh) {
+    constexpr int L = RA * (MIDS * ... * 1) * RZ;
+    constexpr int LZ = L / RZ;                       // first-pass butterflies per column (their stride)
+    constexpr int QF = (LZ * W + NT - 1) / NT;       // first-pass butterflies per thread
+    constexpr int LA = L / RA;                       // last-pass butterflies per column
+    constexpr int QL = (LA * W + NT - 1) / NT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* buf = (cpx<T>*)smem_raw;        // L * W
+    cpx<T>* tw = buf + (size_t)L * W;       // L
+    int32_t* ipos = (int32_t*)(tw + L);     // L
+    const int tid0 = threadIdx.x;
+    const int G = (int)gridDim.x;           // a multiple of 8
+    const int first = (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+    const uint32_t bytes = (uint32_t)(((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>));
+    const auto rsrc = buf_rsrc(data, bytes);
+    for (int k = tid0; k < L; k += NT) {
+        tw[k] = tw_g[k];
+        ipos[perm_g[k]] = k;
+    }
+    cpx<T> r[QF * RZ];
+    auto load = [&](int panel, int tid) {
+        const int j0 = panel * W;
+        const int w = panel < npanels ? min(W, N - j0) : 0;
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+            const int idx = tid + NT * q;
+            const int v = idx % W, j = idx / W;
+            const bool ok = idx < LZ * W && v < w;
+#pragma unroll
+            for (int rr = 0; rr < RZ; ++rr) {
+                const uint32_t off =
+                    ok ? (uint32_t)(((int64_t)(j + LZ * rr) * ld + j0 + v) * (int64_t)sizeof(cpx<T>)) : kBufSkip;
+                r[q * RZ + rr] = buf_load<T>(rsrc, off);
+            }
+        }
+    };
+    // first DIF pass (radix RZ, Ns = L / RZ, twiddle w_L^{j r}) into LDS
+    auto fill = [&](int tid) {
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+            const int idx = tid + NT * q;
+            if (idx < LZ * W) {
+                const int v = idx % W, j = idx / W;
+                cpx<T> a[RZ];
+#pragma unroll
+                for (int rr = 0; rr < RZ; ++rr) a[rr] = r[q * RZ + rr];
+                Dft<T, RZ, INV>::run(a);
+                if (j) {
+#pragma unroll
+                    for (int rr = 1; rr < RZ; ++rr) {
+                        const cpx<T> wv = tw[j * rr];
+                        a[rr] = INV ? cmulc(a[rr], wv) : cmul(a[rr], wv);
+                    }
+                }
+#pragma unroll
+                for (int rr = 0; rr < RZ; ++rr) buf[(j + LZ * rr) * W + v] = a[rr];
+            }
+        }
+    };
+    // last DIF pass (radix RA, Ns = 1) and the store
+    auto last = [&](int panel, int tid) {
+        const int j0 = panel * W;
+        const int w = min(W, N - j0);
+#pragma unroll
+        for (int q = 0; q < QL; ++q) {
+            const int idx = tid + NT * q;
+            const int v = idx % W;
+            const bool ok = idx < LA * W && v < w;
+            const int g = ok ? idx / W : 0;
+            cpx<T> a[RA];
+#pragma unroll
+            for (int rr = 0; rr < RA; ++rr) a[rr] = buf[(g * RA + rr) * W + v];
+            auto sink = [&](int k, cpx<T> y) {
+                int row = ipos[g * RA + k] - sh;
+                if (row < 0) row += L;
+                const uint32_t off =
+                    ok ? (uint32_t)(((int64_t)row * ld + j0 + v) * (int64_t)sizeof(cpx<T>)) : kBufSkip;
+                buf_store(INV ? scl(y, scale) : y, rsrc, off);
+            };
+            Dft<T, RA, INV>::run_sink(a, sink);
+        }
+    };
+    load(first, tid0);
+    fill(tid0);
+    __syncthreads();
+    for (int panel = first; panel < npanels; panel += G) {
+        int tid = tid0;
+        asm volatile("" : "+v"(tid));
+        load(panel + G, tid);  // in flight during the passes (all masked past the last panel)
+        DifSeq<T, INV, MIDS...>::run(buf, L, RA, W, 1, W, tw, tid, NT);
+        last(panel, tid);
+        __syncthreads();  // every lane has read this panel from LDS
+        fill(tid);
+        __syncthreads();
+    }
+}
+
 // Workgroups of the persistent kernel: one per CU (the LDS allows no more),
 // a multiple of 8 so every XCD gets the same count.
 static int pipe_workgroups(int npanels) {
@@ -238,6 +351,31 @@ static int fixed_sequence(const FftPlan& p) {
     if (is({71, 5, 4})) return 2;
     if (is({19, 5, 8, 2})) return 3;
     return 0;
+}
+
+template <typename T, bool INV, int W, int RA, int RZ, int... MIDS>
+static void launch_az_reg(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
+    constexpr int NT = 512;
+    constexpr int L = RA * (MIDS * ... * 1) * RZ;
+    if (((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>) >= ((int64_t)1 << 31)) {
+        // 32-bit buffer offsets: larger arrays take the one-panel kernel
+        launch_az_fixed<T, INV, W, RA, MIDS..., RZ>(s, fl, data, ld, N, scale, sh);
+        return;
+    }
+    const int npanels = (N + W - 1) / W;
+    const size_t lds = (size_t)L * W * sizeof(cpx<T>) + (size_t)L * (sizeof(cpx<T>) + sizeof(int32_t));
+    hipLaunchKernelGGL((az_fft_reg_kernel<T, INV, NT, W, RA, RZ, MIDS...>), dim3((unsigned)pipe_workgroups(npanels)),
+                       dim3(NT), lds, s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale, sh);
+}
+
+template <typename T, bool INV, int W>
+static bool az_reg(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
+    switch (fixed_sequence(fl.plan)) {
+    case 1: launch_az_reg<T, INV, W, 37, 8, 5>(s, fl, data, ld, N, scale, sh); return true;
+    case 2: launch_az_reg<T, INV, W, 71, 4, 5>(s, fl, data, ld, N, scale, sh); return true;
+    case 3: launch_az_reg<T, INV, W, 19, 2, 5, 8>(s, fl, data, ld, N, scale, sh); return true;
+    default: return false;
+    }
 }
 
 // Plans with a compiled radix sequence (the IW1/IW2/IW3 burst lengths).
@@ -337,11 +475,12 @@ static int launch_az_generic(hipStream_t s, int prec, const DevFft& fl, void* da
     return (int)hipGetLastError();
 }
 
-int azimuth_fft_variants(const DevFft& fl) { return fixed_sequence(fl.plan) ? 4 : 1; }
+int azimuth_fft_variants(const DevFft& fl) { return fixed_sequence(fl.plan) ? 5 : 1; }
 
 // Variants: with a compiled radix sequence 0 = 64-byte panels, 1 = 32-byte
 // panels (twice the workgroups per CU), 2 = the generic kernel, 3 = 64-byte
-// panels, persistent with the next panel's loads overlapping the passes;
+// panels, persistent with the next panel's loads overlapping the passes,
+// 4 = as 3 with the first and last passes in registers (az_fft_reg_kernel);
 // otherwise only the generic kernel.
 int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
                        int inverse, int variant, int shift) {
@@ -350,7 +489,10 @@ int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, in
         if (prec == S1G_F32) {
             const float sc = (float)(1.0 / (double)L);
             cpx<float>* d = (cpx<float>*)data;
-            if (variant == 3)
+            if (variant == 4)
+                inverse ? az_reg<float, true, 8>(s, fl, d, ld, N, sc, sh)
+                        : az_reg<float, false, 8>(s, fl, d, ld, N, 1.0f, sh);
+            else if (variant == 3)
                 inverse ? az_fixed<float, true, 8, true>(s, fl, d, ld, N, sc, sh)
                         : az_fixed<float, false, 8, true>(s, fl, d, ld, N, 1.0f, sh);
             else if (variant == 1)
@@ -362,7 +504,10 @@ int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, in
         } else {
             const double sc = 1.0 / (double)L;
             cpx<double>* d = (cpx<double>*)data;
-            if (variant == 3)
+            if (variant == 4)
+                inverse ? az_reg<double, true, 4>(s, fl, d, ld, N, sc, sh)
+                        : az_reg<double, false, 4>(s, fl, d, ld, N, 1.0, sh);
+            else if (variant == 3)
                 inverse ? az_fixed<double, true, 4, true>(s, fl, d, ld, N, sc, sh)
                         : az_fixed<double, false, 4, true>(s, fl, d, ld, N, 1.0, sh);
             else if (variant == 1)

@@ -65,7 +65,7 @@ def test_azimuth_variants(vctx, L, inverse):
     for prec, dt, tol in ((F64, torch.complex128, 1e-13), (F32, torch.complex64, 2e-6)):
         x0 = torch.from_numpy(a).to(dt).cuda()
         n = _variant_count(vctx, stage, lambda: vctx.azimuth_fft(x0.clone(), inverse=inverse))
-        assert n == 4
+        assert n == 5
         for v in range(n):
             vctx.force_variant(stage, v)
             x = x0.clone()
@@ -77,20 +77,22 @@ def test_azimuth_variants(vctx, L, inverse):
 
 @pytest.mark.parametrize("inverse", [False, True])
 def test_azimuth_persistent_many_panels(vctx, inverse):
-    """The persistent variant (3) walks several panels per workgroup, the
-    last one partial: 4500 columns = 563 panels of 8 (FP32) / 1125 of 4 (FP64)."""
+    """The persistent variants (3; 4 with the first and last passes in
+    registers) walk several panels per workgroup, the last one partial:
+    4500 columns = 563 panels of 8 (FP32) / 1125 of 4 (FP64)."""
     L, N = 1480, 4500
     rng = np.random.default_rng(11 + inverse)
     a = rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N))
     ref = oracle.azimuth_ifft(a) if inverse else oracle.azimuth_fft(a)
     stage = "az_ifft" if inverse else "az_fft"
     for prec, dt, tol in ((F64, torch.complex128, 1e-13), (F32, torch.complex64, 2e-6)):
-        vctx.force_variant(stage, 3)
-        x = torch.from_numpy(a).to(dt).cuda()
-        vctx.azimuth_fft(x, inverse=inverse)
-        torch.cuda.synchronize()
-        assert vctx.last_variant(stage) == (3, 4)
-        assert rel_l2(x.cpu().numpy(), ref) < tol, prec
+        for v in (3, 4):
+            vctx.force_variant(stage, v)
+            x = torch.from_numpy(a).to(dt).cuda()
+            vctx.azimuth_fft(x, inverse=inverse)
+            torch.cuda.synchronize()
+            assert vctx.last_variant(stage) == (v, 5)
+            assert rel_l2(x.cpu().numpy(), ref) < tol, (prec, v)
     vctx.force_variant(stage, -1)
 
 
