@@ -102,15 +102,21 @@ struct PacketWords {
     int nbytes;
     int wlim;
 
+    // Word i from the aligned words lo = al[i], hi = al[i + 1] (zero when
+    // past last_al).
+    __device__ __forceinline__ uint32_t word_of(uint32_t lo, uint32_t hi, int i) const {
+        const int valid = nbytes - 4 * i;  // bytes of this word inside the packet
+        if (valid <= 0) return 0u;
+        const uint64_t v = ((uint64_t)hi << 32) | lo;
+        uint32_t be = __builtin_bswap32((uint32_t)(v >> (8 * off)));
+        if (valid < 4) be &= ~(0xFFFFFFFFu >> (8 * valid));
+        return be;
+    }
     __device__ __forceinline__ uint32_t global_word(int i) const {
         if (4 * i >= nbytes) return 0u;
         const uint32_t lo = (i <= last_al) ? al[i] : 0u;
         const uint32_t hi = (off && i + 1 <= last_al) ? al[i + 1] : 0u;
-        const uint64_t v = ((uint64_t)hi << 32) | lo;
-        uint32_t be = __builtin_bswap32((uint32_t)(v >> (8 * off)));
-        const int valid = nbytes - 4 * i;  // bytes of this word inside the packet
-        if (valid < 4) be &= ~(0xFFFFFFFFu >> (8 * valid));
-        return be;
+        return word_of(lo, hi, i);
     }
     // With the whole packet staged (ALL) the reads need no bounds test: the
     // walk reads at most 67 words past the data (lane 63 of a block starting
@@ -452,8 +458,27 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     }
     if (st == S1G_PKT_OK) {
         // ---- stage the leading words (all of them for packets up to the cap)
+        // Raw buffer loads over the aligned words (zero past last_al): no
+        // branch around a load, so each thread issues all kStageBatch pairs of
+        // a batch before the first wait.
         const int n_stage = min(lds_words, words.wlim);
-        for (int i = tid; i < n_stage; i += 128) staged[i] = words.global_word(i);
+        const __amdgpu_buffer_rsrc_t prs = buf_rsrc(words.al, 4u * (uint32_t)(words.last_al + 1));
+        constexpr int kStageBatch = 8;
+        for (int b = 0; b < n_stage; b += 128 * kStageBatch) {
+            uint32_t lo[kStageBatch], hi[kStageBatch];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int i = b + tid + 128 * u;
+                const uint32_t o = i < n_stage ? 4u * (uint32_t)i : kBufSkip;
+                lo[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(prs, (int)o, 0, 0);
+                hi[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(prs, (int)(o + 4u), 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int i = b + tid + 128 * u;
+                if (i < n_stage) staged[i] = words.word_of(lo[u], hi[u], i);
+            }
+        }
     }
     if (tid == 0) {
         s_st = st;
