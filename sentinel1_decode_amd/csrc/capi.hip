@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <new>
@@ -60,6 +61,9 @@ struct s1g_ctx {
     bool profiling = false;
     bool capturing = false;
     std::vector<hipEvent_t> event_pool;
+    // batches: the stream decodes run on, and ordering events
+    hipStream_t side = nullptr;
+    std::vector<hipEvent_t> sync_events;
     struct Mark {
         int stage;
         hipEvent_t a, b;
@@ -475,6 +479,8 @@ int s1g_destroy(s1g_ctx* c) {
         hipEventDestroy(m.b);
     }
     for (auto e : c->event_pool) hipEventDestroy(e);
+    if (c->side) hipStreamSynchronize(c->side);
+    for (auto e : c->sync_events) hipEventDestroy(e);
     for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (auto g : c->graph_objs) hipGraphDestroy(g);
     for (auto& kv : c->ffts) {
@@ -494,6 +500,7 @@ int s1g_destroy(s1g_ctx* c) {
     for (auto& kv : c->blue_filters) hipFree(kv.second);
     hipFree(c->dsp_scal);
     if (c->own) hipStreamDestroy(c->own);
+    if (c->side) hipStreamDestroy(c->side);
     delete c;
     return S1G_OK;
 }
@@ -870,7 +877,9 @@ extern "C" int s1g_sinc_interpolate(s1g_ctx* c, int prec, const void* signal, in
                         "sinc interpolation");
 }
 
-static int burst_chain(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
+// A burst's chain in two parts: decode (with ECCM, through the workspace)
+// into b.out, then range compression onward (workspace and b.out).
+static int burst_decode(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
     const int L = b.range.n_lines, N = b.range.n_samples;
     if (L <= 1 || N <= 0) return fail(c, S1G_EARG, "burst needs >= 2 lines%s");
     if (b.azimuth.n_lines != L || b.azimuth.n_samples != N) return fail(c, S1G_EARG, "descriptor shapes disagree%s");
@@ -888,14 +897,107 @@ static int burst_chain(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
         rc = s1g_decode_fdbaq(c, b.user_data, b.ud_offsets, b.ud_lengths, b.num_quads, L, b.max_user_data_bytes,
                               b.max_num_quads, prec, b.out, b.ld_out, b.status);
     }
-    if (rc) return rc;
-    rc = do_range(c, b.range, prec, b.out, b.ld_out, c->ws, N);
+    return rc;
+}
+
+static int burst_focus(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
+    const int L = b.range.n_lines, N = b.range.n_samples;
+    int rc = do_range(c, b.range, prec, b.out, b.ld_out, c->ws, N);
     if (rc) return rc;
     rc = do_azimuth_fft(c, prec, L, N, c->ws, N, 0);
     if (rc) return rc;
     rc = do_rcmc(c, b.azimuth, prec, c->ws, N, b.out, b.ld_out);
     if (rc) return rc;
     return do_azimuth_fft(c, prec, L, N, b.out, b.ld_out, 1);
+}
+
+static int burst_chain(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
+    int rc = burst_decode(c, b, prec);
+    return rc ? rc : burst_focus(c, b, prec);
+}
+
+// Bytes [lo, hi) a burst's decode and chain write: the SLC rows and the
+// packet status words.
+static void burst_extent(const s1g_burst_desc& b, int prec, uintptr_t r[2][2]) {
+    const size_t L = (size_t)std::max(b.range.n_lines, 0), N = (size_t)std::max(b.range.n_samples, 0);
+    r[0][0] = (uintptr_t)b.out;
+    r[0][1] = r[0][0] + (L ? esize(prec) * ((L - 1) * (size_t)b.ld_out + N) : 0);
+    r[1][0] = (uintptr_t)b.status;
+    r[1][1] = r[1][0] + sizeof(int32_t) * L;
+}
+
+// A batch may decode burst i + 1 beside the focusing of burst i when no
+// decode goes through the shared workspace (no ECCM) and no burst writes
+// memory another one writes.
+static bool batch_can_overlap(const s1g_burst_desc* d, int n, int prec) {
+    if (n < 2) return false;
+    std::vector<std::array<uintptr_t, 2>> ext;
+    for (int i = 0; i < n; ++i) {
+        if (d[i].use_eccm) return false;
+        uintptr_t r[2][2];
+        burst_extent(d[i], prec, r);
+        ext.push_back({r[0][0], r[0][1]});
+        ext.push_back({r[1][0], r[1][1]});
+    }
+    for (size_t i = 0; i < ext.size(); ++i)
+        for (size_t j = i + 1; j < ext.size(); ++j)
+            if (ext[i][0] < ext[j][1] && ext[j][0] < ext[i][1]) return false;
+    return true;
+}
+
+static int sync_event(s1g_ctx* c, size_t i, hipEvent_t* e) {
+    while (c->sync_events.size() <= i) {
+        hipEvent_t ev = nullptr;
+        HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        c->sync_events.push_back(ev);
+    }
+    *e = c->sync_events[i];
+    return S1G_OK;
+}
+
+// The bursts' chains in order on c->stream.  When batch_can_overlap holds,
+// the decodes run on a second stream (default priority), each focusing
+// waiting on its own burst's decode only, so the decode of burst i + 1
+// (latency-bound, a few waves per SIMD) runs beside the focusing of burst i.
+// Both streams order after the work already on c->stream, and c->stream
+// orders after the last decode again before returning, so the batch stays
+// one ordered unit on c->stream (and one graph when captured).
+static int batch_chains(s1g_ctx* c, const s1g_burst_desc* d, int n, int prec) {
+    if (!batch_can_overlap(d, n, prec)) {
+        for (int i = 0; i < n; ++i) {
+            int rc = burst_chain(c, d[i], prec);
+            if (rc) return rc;
+        }
+        return S1G_OK;
+    }
+    if (!c->side) HIPCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    hipEvent_t fork;
+    int rc = sync_event(c, 0, &fork);
+    if (rc) return rc;
+    hipStream_t main_s = c->stream;
+    HIPCHK(c, hipEventRecord(fork, main_s));
+    HIPCHK(c, hipStreamWaitEvent(c->side, fork, 0));
+    for (int i = 0; i < n && !rc; ++i) {
+        hipEvent_t dec;
+        rc = sync_event(c, 1 + (size_t)i, &dec);
+        if (rc) break;
+        c->stream = c->side;
+        rc = burst_decode(c, d[i], prec);
+        c->stream = main_s;
+        if (rc) break;
+        HIPCHK(c, hipEventRecord(dec, c->side));
+        HIPCHK(c, hipStreamWaitEvent(main_s, dec, 0));
+        rc = burst_focus(c, d[i], prec);
+    }
+    c->stream = main_s;
+    if (rc) {
+        // the side stream rejoins c->stream on the error path too
+        hipEvent_t join;
+        if (!sync_event(c, (size_t)n + 1, &join) && hipEventRecord(join, c->side) == hipSuccess)
+            (void)hipStreamWaitEvent(main_s, join, 0);
+        return rc;
+    }
+    return S1G_OK;  // c->stream already waits on the last decode
 }
 
 int s1g_focus_burst(s1g_ctx* c, const s1g_burst_desc* b, int prec) {
@@ -952,13 +1054,7 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
     int rc = ensure(c, &c->ws, &c->ws_bytes, ws_need);
     if (!rc) rc = ensure(c, (void**)&c->cols, &c->cols_n, cols_need);
     if (rc) return rc;
-    if (!(flags & S1G_BATCH_GRAPH)) {
-        for (int i = 0; i < n; ++i) {
-            rc = burst_chain(c, descs[i], prec);
-            if (rc) return rc;
-        }
-        return S1G_OK;
-    }
+    if (!(flags & S1G_BATCH_GRAPH)) return batch_chains(c, descs, n, prec);
     // one hipGraph per distinct descriptor set (and forced variants)
     std::string key((const char*)descs, sizeof(s1g_burst_desc) * (size_t)n);
     key.append((const char*)&prec, sizeof prec);
@@ -979,7 +1075,7 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
             c->stream = user;
             return hip_fail(c, e, "hipStreamBeginCapture");
         }
-        for (int i = 0; i < n && !rc; ++i) rc = burst_chain(c, descs[i], prec);
+        rc = batch_chains(c, descs, n, prec);
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(c->own, &g);
         c->capturing = false;

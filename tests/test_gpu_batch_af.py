@@ -51,6 +51,23 @@ def test_batch_graph_matches_eager(ctx, uneven_bursts, prec):
         assert torch.equal(o, e)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_shared_output_runs_in_order(ctx, uneven_bursts, graph):
+    """Two descriptors that write the same SLC and status words (the same
+    burst twice) are not decoded beside each other's focusing: the batch
+    runs the chains one after the other and ends with the burst's own
+    image."""
+    db = uneven_bursts[1]
+    eager = db.focus(ctx, F32)
+    out = torch.zeros((db.n_lines, db.n_samples), dtype=torch.complex64, device="cuda")
+    d = db.burst_desc(out)
+    for rep in range(2):
+        out.zero_()
+        ctx.focus_batch([d, d], F32, graph=graph)
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), rep
+
+
 def _af_problem(L=256, N=96, seed=3):
     rng = np.random.default_rng(seed)
     rd = (rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N)))
