@@ -233,7 +233,13 @@ int s1g_focus_burst(s1g_ctx* ctx, const s1g_burst_desc* desc, int prec);
 
 /* Several bursts (uneven sizes allowed) in one submission.  flags:
  *   S1G_BATCH_GRAPH  capture the whole batch as one hipGraph and replay it
- *                    (the graph is cached per descriptor set).             */
+ *                    (the graph is cached per descriptor set).
+ * When no burst uses ECCM and no two bursts' SLC rows or status words
+ * overlap, the decode of burst i+1 runs on a second stream of the context
+ * beside the focusing of burst i (each burst's focusing waits only on its
+ * own decode); otherwise the chains run in order.  Either way the batch is
+ * ordered on the context's stream as one unit.  A burst's SLC must not
+ * alias another burst's packet data.                                       */
 #define S1G_BATCH_GRAPH 1
 int s1g_focus_batch(s1g_ctx* ctx, const s1g_burst_desc* descs, int n_bursts, int prec, int flags);
 
