@@ -68,6 +68,30 @@ def test_batch_shared_output_runs_in_order(ctx, uneven_bursts, graph):
         assert torch.equal(out, eager), rep
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_with_eccm_matches_single_bursts(ctx, uneven_bursts, graph):
+    """A batch in which one burst runs ECCM (its decode goes through the
+    shared workspace, so the chains run in order) gives each burst the image
+    and flags of its own single call."""
+    a, b = uneven_bursts[1], uneven_bursts[0]
+    ed = capi.EccmDesc(a.n_lines, a.n_samples, 128, 64, 1.0e3, 4000.0)
+    fa = torch.zeros(a.n_lines, dtype=torch.uint8, device="cuda")
+    ea = a.focus(ctx, F32, eccm=ed, eccm_flags=fa)
+    eb = b.focus(ctx, F32)
+    torch.cuda.synchronize()
+    outs = [torch.zeros_like(ea), torch.zeros_like(eb)]
+    flags = torch.full_like(fa, 7)
+    descs = [a.burst_desc(outs[0], eccm=ed, eccm_flags=flags), b.burst_desc(outs[1])]
+    for rep in range(2):
+        for o in outs:
+            o.zero_()
+        ctx.focus_batch(descs, F32, graph=graph)
+        torch.cuda.synchronize()
+        assert torch.equal(outs[0], ea), rep
+        assert torch.equal(outs[1], eb), rep
+        assert torch.equal(flags, fa), rep
+
+
 def _af_problem(L=256, N=96, seed=3):
     rng = np.random.default_rng(seed)
     rd = (rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N)))
