@@ -238,12 +238,12 @@ __device__ __forceinline__ int select_bit(uint32_t w, int j) {
 //      nsym-th symbol; the block ends where its next recorded symbol starts,
 //      or at its exit.
 // Regions repeat when one does not hold all nsym symbols.  delta <= 32.
-// *hdr receives the 8 bits that follow the end (the next block's BRC or
-// THIDX field).  Reads stay below word `wlim`; once `start` passes `nbits`
+// With want_hdr (IE, QE) *hdr receives the 8 bits that follow the end (the
+// next block's BRC or THIDX field), else 0.  Reads stay below word `wlim`; once `start` passes `nbits`
 // the block cannot end inside the packet and the walk stops there.
 template <int BRC, bool ALL>
 __device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, int nsym, int delta, int lane,
-                          uint32_t* hdr) {
+                          bool want_hdr, uint32_t* hdr) {
     constexpr bool B4 = BRC == 4;
     for (;;) {
         if (start > nbits) {
@@ -323,8 +323,12 @@ __device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, i
             // block ends at its (k+1)-th start, or at its exit when k == cnt
             const int k = nsym - (P - cnt);
             const int endrel = (k >= 1 && k < cnt) ? entry + select_bit(live, k + 1) : ex;
-            const uint32_t h = (uint32_t)(win64(w, min(zero + endrel, 95)) >> 56);
-            *hdr = (uint32_t)__builtin_amdgcn_readlane((int)h, ls);
+            if (want_hdr) {  // uniform: IE and QE blocks are followed by a header field
+                const uint32_t h = (uint32_t)(win64(w, min(zero + endrel, 95)) >> 56);
+                *hdr = (uint32_t)__builtin_amdgcn_readlane((int)h, ls);
+            } else {
+                *hdr = 0;
+            }
             return start + ls * delta + __builtin_amdgcn_readlane(endrel, ls);
         }
         nsym -= total;
@@ -494,7 +498,8 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     auto walk_channel = [&](int ch) __attribute__((always_inline)) {
         // header field of the channel's first block from the packet; later
         // ones come out of the walk of the block before
-        uint32_t hdr = (ch == 0 || ch == 2) ? peek_bits(words, pos, 8) : 0u;
+        const bool hdr_field = ch == 0 || ch == 2;
+        uint32_t hdr = hdr_field ? peek_bits(words, pos, 8) : 0u;
         for (int b = 0; b < nb && st == S1G_PKT_OK; ++b) {
             int brc;
             if (ch == 0) {
@@ -529,11 +534,11 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
             const int delta = min(max((est + 63) / 64 + 1, 2), 32);
             int end;
             switch (brc) {
-                case 0: end = walk_block<0, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
-                case 1: end = walk_block<1, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
-                case 2: end = walk_block<2, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
-                case 3: end = walk_block<3, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
-                default: end = walk_block<4, ALL>(words, nbits, pos, nsym, delta, lane, &hdr); break;
+                case 0: end = walk_block<0, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
+                case 1: end = walk_block<1, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
+                case 2: end = walk_block<2, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
+                case 3: end = walk_block<3, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
+                default: end = walk_block<4, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
             }
             prev_len = (int)(end - pos);
             pos = end;
