@@ -195,18 +195,18 @@ __device__ __forceinline__ uint64_t walk_window(uint64_t bits) { return ~(bits <
 __device__ __forceinline__ int from_left(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, false); }
 
 // Inclusive prefix sum across the 64 lanes: row_shr 1, 2, 4, 8 inside each
-// row of 16 lanes, then the totals of the lower rows (read from lanes 15,
-// 31, 47) are added.
+// row of 16 lanes, then row_bcast:15 adds lane 15's sum into the next row
+// (rows 1 and 3) and row_bcast:31 adds lane 31's into rows 2 and 3; rows
+// outside a DPP's row mask add the `old` operand, 0.
 __device__ __forceinline__ int wave_inclusive_sum(int x, int lane) {
+    (void)lane;
     x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
     x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
     x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
     x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
-    const int r0 = __builtin_amdgcn_readlane(x, 15);
-    const int r1 = __builtin_amdgcn_readlane(x, 31);
-    const int r2 = __builtin_amdgcn_readlane(x, 47);
-    const int row = lane >> 4;
-    return x + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);
+    return x;
 }
 
 // Position of the j-th (1-based, 1 <= j <= popc(w)) set bit of w: a binary
