@@ -547,18 +547,30 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
         pos = (pos + 15) & ~15;
     };
     if (wave == 0 && st == S1G_PKT_OK) {
+        __builtin_amdgcn_s_setprio(2);
         for (int ch = 0; ch < 3 && st == S1G_PKT_OK; ++ch) walk_channel(ch);
+        __builtin_amdgcn_s_setprio(0);
         if (lane == 0 && st != S1G_PKT_OK) s_st = st;
     }
     __syncthreads();
     st = s_st;
+    // IE/IO/QE values: wave 1 decodes items [0, n2) beside the QO walk, the
+    // rest fill the lanes the QO blocks leave free in the last round (128
+    // lanes for nb QO blocks), so that round stays one round
+    const int n2 = min(3 * nb, max(4 * nb - 128, 0));
     if (st == S1G_PKT_OK) {
         if (wave == 0) {
+            // the QO walk is the packet's critical path; the value decodes
+            // beside it (this packet's and other packets') have slack, so
+            // the walking wave takes issue priority on its SIMD meanwhile
+            __builtin_amdgcn_s_setprio(2);
             walk_channel(3);
+            __builtin_amdgcn_s_setprio(0);
             if (lane == 0 && st != S1G_PKT_OK) s_st_qo = st;
         } else {
-            // ---- phase 2 (wave 1): IE, IO, QE while wave 0 walks QO
-            for (int item = lane; item < 3 * nb; item += 64)
+            // ---- phase 2 (wave 1): IE, IO, QE items [0, n2) while wave 0
+            // walks QO
+            for (int item = lane; item < n2; item += 64)
                 decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, item % 3, item / 3, orow);
         }
     }
@@ -570,8 +582,14 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
         if (tid == 0) status[p] = st;
         return;
     }
-    // ---- phase 2 (both waves): QO
-    for (int b = tid; b < nb; b += 128) decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, 3, b, orow);
+    // ---- phase 3 (both waves): QO, then the IE/IO/QE items from n2 on
+    const int n3 = nb + (3 * nb - n2);
+    for (int t = tid; t < n3; t += 128) {
+        const int item = n2 + (t - nb);
+        const int ch = t < nb ? 3 : item % 3;
+        const int b = t < nb ? t : item / 3;
+        decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, ch, b, orow);
+    }
     if (tid == 0) status[p] = S1G_PKT_OK;
 }
 
