@@ -406,9 +406,12 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
 
 // Two wavefronts per packet.  Wave 0 walks the block structure of IE, IO
 // and QE; QE carries the THIDX every channel's reconstruction needs, so at
-// that barrier wave 1 starts decoding the three channels while wave 0 walks
-// QO; after the second barrier both decode QO.  Only workgroup barriers
-// hand work over, and every wave passes each of them once.
+// that barrier wave 1 starts decoding the three channels' values while wave
+// 0 walks QO; after the second barrier both waves decode QO and the rest of
+// the three channels' items.  The walk is the packet's critical path, so the
+// walking wave raises its issue priority (s_setprio) over the value decodes
+// sharing its SIMD.  Only workgroup barriers hand work over, and every wave
+// passes each of them once.
 template <typename T, bool ALL>
 __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
                                                            const uint64_t* __restrict__ offs,
