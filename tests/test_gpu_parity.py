@@ -132,6 +132,22 @@ def test_decode_bitexact_large_packets(ctx, prec):
     assert _bits_equal(out.cpu().numpy(), ref, prec)
 
 
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_decode_bitexact_more_than_128_blocks(ctx, prec):
+    """36,000-sample lines: 141 BAQ blocks per channel, more than the 128
+    lanes of the final value round, so every IE/IO/QE item stays with wave 1
+    and QO takes two rounds; bit-exact against the oracle."""
+    spec = synth.config1(n_lines=6, n_samples=36000, n_targets=20, seed=91)
+    burst = synth.make_burst(spec)
+    assert ((np.asarray(burst.num_quads) + 127) // 128 > 128).all()
+    db = pipeline.from_synth(burst)
+    out, st = db.decode(ctx, prec)
+    torch.cuda.synchronize()
+    ref, rst = oracle.decode_packets(burst.stream, burst.ud_offsets, burst.ud_lengths, burst.num_quads)
+    assert (rst == 0).all() and (st.cpu().numpy() == 0).all()
+    assert _bits_equal(out.cpu().numpy(), ref, prec)
+
+
 @pytest.mark.parametrize("prec,tol", [(F64, 1e-11), (F32, 2e-6)])
 def test_range_compress(ctx, small_burst, small_chain, prec, tol):
     db = pipeline.from_synth(small_burst)
