@@ -31,6 +31,17 @@ METRIC = "focused Msamples/s L0->SLC, FP32 vs FP64 GPU path, %HBM peak, rel-L2<=
 HBM_PEAK_GBS = 8000.0
 
 
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def peak_phase_err(a, b, n=64):
+    """Largest phase difference over the n brightest pixels of b (the
+    point-target peaks)."""
+    flat = np.argsort(np.abs(b).ravel())[-n:]
+    return float(np.abs(np.angle(a.ravel()[flat] * np.conj(b.ravel()[flat]))).max())
+
+
 def stage_bytes(prec_bytes, L, N, b_pkt):
     """Algorithmic bytes per launch: one read of the stage's logical input
     plus one write of its logical output (twiddles, filters and tables count
@@ -93,6 +104,7 @@ def run_gpu(burst, steps, warmup):
               + ", ".join(f"{s}={v:.3f}ms" for s, v in res[name]["stages_ms"].items()), file=sys.stderr, flush=True)
     a = outs["fp64"].cpu().numpy()
     b = outs["fp32"].cpu().numpy()
+    res["_slc64"], res["_slc32"] = a, b
     res["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(b - a) / np.linalg.norm(a))
     # peak point-target phase error: brightest 64 pixels of the FP64 image
     flat = np.argsort(np.abs(a).ravel())[-64:]
@@ -104,6 +116,17 @@ def run_gpu(burst, steps, warmup):
     torch.cuda.synchronize()
     res["_dec64"] = dec64.cpu().numpy()
     res["_dec32"] = dec32.cpu().numpy()
+    # the FP32 chain stage by stage (each stage's output kept), for the
+    # per-stage error against the oracle's stage outputs
+    rc = ctx.range_compress(dec32, db.geo.range_desc())
+    res["_stages32"] = {"rc": rc.cpu().numpy()}
+    ctx.azimuth_fft(rc, inverse=False)
+    res["_stages32"]["rd"] = rc.cpu().numpy()
+    mf = ctx.rcmc_matched_filter(rc, db.azimuth_desc())
+    res["_stages32"]["mf"] = mf.cpu().numpy()
+    ctx.azimuth_fft(mf, inverse=True)
+    res["_stages32"]["slc"] = mf.cpu().numpy()
+    del dec64, dec32, rc, mf
     ctx.close()
     return res, L, N
 
@@ -153,14 +176,39 @@ def measured_traffic(kernel_prefix, L, N):
     return None
 
 
-def run_cpu_baseline(burst):
-    """The oracle (CPU FP64 restatement) on the same packets, full chain."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(burst, threads):
+    """The oracle (CPU FP64 restatement, its own mixed-radix FFT, not FFTW)
+    on the same packets, full chain: one warm-up then the median of three
+    runs on `threads` OpenMP threads, and one run on a single thread.
+    Returns (stage outputs of the last run, median seconds, 1-thread
+    seconds)."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
+    import oracle
     from chain_ref import oracle_chain
+    oracle.set_threads(threads)
+    out = oracle_chain(burst)  # warm-up (FFT plans, page faults)
+    secs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = oracle_chain(burst)
+        secs.append(time.perf_counter() - t0)
+    oracle.set_threads(1)
     t0 = time.perf_counter()
-    out = oracle_chain(burst)
-    t1 = time.perf_counter()
-    return out, t1 - t0
+    oracle_chain(burst)
+    one = time.perf_counter() - t0
+    oracle.set_threads(threads)
+    return out, float(np.median(secs)), one
 
 
 def _timed(fn, steps, warmup):
@@ -399,15 +447,31 @@ def main():
 
     cpu = None
     dec_mismatch = None
+    vs_oracle = None
     threads = int(os.environ["OMP_NUM_THREADS"])
     if not args.no_cpu:
-        ref, secs = run_cpu_baseline(burst)
+        ref, secs, secs1 = run_cpu_baseline(burst, threads)
         dec = ref["decoded"]
         dec_mismatch = int((res["_dec64"].view(np.uint64) != dec.view(np.uint64)).sum()
                            + (res["_dec32"].view(np.uint32) != dec.astype(np.complex64).view(np.uint32)).sum())
         cpu = {"value": L * N / secs / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
                "sample": f"the same {L}x{N} burst, full FP64 chain (decode, range, azimuth FFT, RCMC+MF, "
-                         f"IFFT) by the oracle on {threads} host threads, {secs:.1f} s"}
+                         f"IFFT) by the oracle (its own mixed-radix FFT, not FFTW) on {threads} OpenMP threads "
+                         f"(the host share of one GPU; nproc {os.cpu_count()}): median of 3 after a warm-up, "
+                         f"{secs:.2f} s",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "one_thread": {"value": L * N / secs1 / 1e6, "seconds": round(secs1, 2)}}
+        vs_oracle = {"rel_l2_fp64_vs_oracle": rel_l2(res["_slc64"], ref["slc"]),
+                     "rel_l2_fp32_vs_oracle": rel_l2(res["_slc32"], ref["slc"]),
+                     "peak_phase_err_fp32_vs_oracle_rad": peak_phase_err(res["_slc32"], ref["slc"]),
+                     "stages_fp32_vs_oracle": {
+                         name: {"rel_l2": rel_l2(res["_stages32"][key], ref[key]),
+                                "peak_phase_err_rad": peak_phase_err(res["_stages32"][key], ref[key])}
+                         for name, key in (("range", "rc"), ("az_fft", "rd"), ("rcmc_mf", "mf"),
+                                           ("az_ifft", "slc"))}}
+        vs_oracle["stages_fp32_vs_oracle"]["decode"] = {
+            "rel_l2": rel_l2(res["_dec32"], dec), "bit_mismatches": dec_mismatch}
+        del ref
     fp32, fp64 = res["fp32"], res["fp64"]
     # dominant kernel of the FP32 chain
     sb = stage_bytes(4, L, N, b_pkt)
@@ -443,6 +507,7 @@ def main():
         "rel_l2_fp32_vs_fp64": res["rel_l2_fp32_vs_fp64"],
         "peak_phase_err_rad": res["peak_phase_err_rad"],
         "decode_mismatches": dec_mismatch,
+        "vs_oracle": vs_oracle,
         "packets_failed": fp32["packets_failed"],
         "stages_ms_fp32": {k: round(v, 4) for k, v in fp32["stages_ms"].items()},
         "stages_ms_fp64": {k: round(v, 4) for k, v in fp64["stages_ms"].items()},

@@ -38,7 +38,7 @@ extern "C" {
 #define S1G_EHIP (-2)        /* HIP runtime error                       */
 #define S1G_EUNSUPPORTED (-3) /* length/shape this build cannot handle   */
 #define S1G_ENOMEM (-4)      /* device allocation failed                */
-#define S1G_EDECODE (-5)     /* >= 1 packet failed; see per-packet status */
+#define S1G_EDECODE (-5)     /* >= 1 packet failed (s1g_check_status)   */
 
 /* per-packet decode status (status[] of s1g_decode_fdbaq) */
 #define S1G_PKT_OK 0
@@ -85,6 +85,16 @@ int s1g_decode_fdbaq(s1g_ctx* ctx, const uint8_t* user_data, const uint64_t* ud_
                      const uint32_t* ud_lengths, const int32_t* num_quads, int n_packets,
                      int max_user_data_bytes, int max_num_quads, int prec, void* out,
                      int64_t ld_out, int32_t* status);
+
+/* Waits for the context's stream, then reads status[0..n_packets) (device
+ * pointer, as written by s1g_decode_fdbaq or a burst chain).  Returns
+ * S1G_EDECODE if any packet failed, with the index of the first one in
+ * *first_failed (may be NULL; -1 when none failed) and its status in the
+ * message of s1g_last_error.  The reference throws from the decode instead
+ * (packet.cpp:795-798, decoding_utils.cpp:69-72,107-119); the decode calls
+ * stay asynchronous and report through status[], and this call is the
+ * synchronising check a caller makes where the reference would have thrown. */
+int s1g_check_status(s1g_ctx* ctx, const int32_t* status, int n_packets, int* first_failed);
 
 /* Host function: length (bits) and magnitude code of the FDBAQ Huffman code
  * word at the top of the 10-bit value `code10` for bit-rate code `brc`

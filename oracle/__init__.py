@@ -10,6 +10,7 @@ function (AndrewPlayer3/sentinel1_decode ``src/`` file:line) it restates.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -40,36 +41,70 @@ def lib():
         so = os.path.join(_HERE, "libs1oracle.so")
         if not os.path.exists(so):
             build()
-        L = C.CDLL(so)
-        L.orc_read_n_bits.argtypes = [_u8, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_uint64)]
-        L.orc_huffman_decode.argtypes = [_u8, C.c_int64, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
-        L.orc_s_value_d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
-        L.orc_s_value_d.restype = C.c_double
-        L.orc_decode_packet_d.argtypes = [_u8, C.c_int64, C.c_int, _dp, _u8, _u8]
-        L.orc_decode_packets.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
-                                         C.c_void_p, C.c_int64, C.c_void_p]
-        L.orc_parse_headers.argtypes = [_u8, C.c_int64, C.c_void_p, C.c_void_p]
-        L.orc_packet_params.argtypes = [C.c_void_p, _dp]
-        L.orc_slant_ranges.argtypes = [C.c_void_p, C.c_int, _dp]
-        L.orc_timing_corrections.argtypes = [C.c_void_p, _dp]
-        L.orc_linspace.argtypes = [C.c_double, C.c_double, C.c_int, _dp]
-        L.orc_fft.argtypes = [_dp, C.c_int, C.c_int]
-        L.orc_fft_many.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_int]
-        L.orc_replica_chirp_length.argtypes = [C.c_double, C.c_double]
-        L.orc_replica_chirp.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
-        L.orc_reference_function.argtypes = [_dp, C.c_int, _dp]
-        L.orc_range_compress.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double,
-                                         C.c_double, C.c_double]
-        L.orc_azimuth_fft.argtypes = [_dp, C.c_int, C.c_int, C.c_int64]
-        L.orc_azimuth_ifft.argtypes = [_dp, C.c_int, C.c_int, C.c_int64]
-        L.orc_rcmc_mf.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
-                                  _dp, _dp, _dp, _dp, C.c_int]
-        L.orc_sinc_interpolate.argtypes = [_dp, C.c_int, C.c_double, C.c_int, _dp]
-        L.orc_eccm.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, _u8]
-        L.orc_af_entropy_grad.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, _dp, _dp]
-        L.orc_af_entropy_grad.restype = C.c_double
-        _LIB = L
+        _LIB = _bind(C.CDLL(so))
     return _LIB
+
+
+def build_mutant(mutation: int, dest_dir: str) -> str:
+    """Compile the oracle with planted mistake `mutation` (see the
+    ORC_MUTATION list in s1_oracle.c) into dest_dir; returns the library path.
+    Used by tests that show a check fails on the mistake it is meant to catch."""
+    so = os.path.join(dest_dir, f"libs1oracle_m{mutation}.so")
+    subprocess.run(["make", "-s", "-C", _HERE, f"MUTANT={so}", f"MUTATION={mutation}", "mutant"], check=True)
+    return so
+
+
+@contextlib.contextmanager
+def using(so_path: str):
+    """Route every wrapper in this module through the library at so_path
+    (an oracle variant from build_mutant) for the duration of the block."""
+    global _LIB
+    saved = lib()
+    _LIB = _bind(C.CDLL(so_path))
+    try:
+        yield
+    finally:
+        _LIB = saved
+
+
+def _bind(L):
+    """Declare the argument types of the oracle's C functions on library L."""
+    L.orc_read_n_bits.argtypes = [_u8, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_uint64)]
+    L.orc_huffman_decode.argtypes = [_u8, C.c_int64, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+    L.orc_s_value_d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.orc_s_value_d.restype = C.c_double
+    L.orc_decode_packet_d.argtypes = [_u8, C.c_int64, C.c_int, _dp, _u8, _u8]
+    L.orc_decode_packets.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.c_void_p, C.c_int64, C.c_void_p]
+    L.orc_parse_headers.argtypes = [_u8, C.c_int64, C.c_void_p, C.c_void_p]
+    L.orc_packet_params.argtypes = [C.c_void_p, _dp]
+    L.orc_slant_ranges.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_timing_corrections.argtypes = [C.c_void_p, _dp]
+    L.orc_linspace.argtypes = [C.c_double, C.c_double, C.c_int, _dp]
+    L.orc_fft.argtypes = [_dp, C.c_int, C.c_int]
+    L.orc_fft_many.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_int]
+    L.orc_replica_chirp_length.argtypes = [C.c_double, C.c_double]
+    L.orc_replica_chirp.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+    L.orc_reference_function.argtypes = [_dp, C.c_int, _dp]
+    L.orc_range_compress.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double,
+                                     C.c_double, C.c_double]
+    L.orc_azimuth_fft.argtypes = [_dp, C.c_int, C.c_int, C.c_int64]
+    L.orc_azimuth_ifft.argtypes = [_dp, C.c_int, C.c_int, C.c_int64]
+    L.orc_rcmc_mf.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double,
+                              _dp, _dp, _dp, _dp, C.c_int]
+    L.orc_sinc_interpolate.argtypes = [_dp, C.c_int, C.c_double, C.c_int, _dp]
+    L.orc_eccm.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, _u8]
+    L.orc_af_entropy_grad.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, _dp, _dp]
+    L.orc_af_entropy_grad.restype = C.c_double
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = C.c_int
+    return L
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads for the oracle's loops (n <= 0: query); returns the
+    count in effect."""
+    return lib().orc_set_threads(int(n))
 
 
 def _table_call(fn):

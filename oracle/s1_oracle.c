@@ -13,6 +13,18 @@
  */
 #include "s1_oracle.h"
 
+/* Planted mistakes, for the tests that show the geometry and azimuth checks
+ * catch them (they build variants with -DORC_MUTATION=k into a scratch
+ * directory).  The oracle itself is built with ORC_MUTATION 0:
+ *   1 sign of 4 PI R D / lambda flipped     2 factor 2 inside D
+ *   3 amplitude weight dropped              4 sign of the t_corr phase flipped
+ *   5 RCMC shift sign flipped               6 f_dc left out of D
+ *   7 az_freqs spacing PRF/L (not PRF/(L-1)) 8 timing-correction j term sign flipped
+ *   9 slant-range linspace without its endpoint */
+#ifndef ORC_MUTATION
+#define ORC_MUTATION 0
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -322,6 +334,16 @@ void orc_packet_params(const int64_t* sec, double* out) {
 }
 
 /* signal_processing.cpp:18-29 */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 void orc_linspace(double start, double end, int size, double* out) {
     double step = (end - start) / (size - 1);
     for (int i = 0; i < size; ++i) out[i] = start + i * step;
@@ -338,6 +360,10 @@ void orc_slant_ranges(const int64_t* sec, int num_ranges, double* out) {
     double rank = (double)sec[S_RANK];
     double near = rank * pri + start + K_DT_SUPPRESSED;
     double far = near + (n_rng / fs);
+#if ORC_MUTATION == 9
+    for (int i = 0; i < num_ranges; i++) out[i] = near * K_C / 2 + i * ((far - near) * K_C / 2) / num_ranges;
+    return;
+#endif
     orc_linspace(near * K_C / 2, far * K_C / 2, num_ranges, out);
 }
 
@@ -349,6 +375,10 @@ void orc_timing_corrections(const int64_t* sec, double* out) {
     double rank = (double)sec[S_RANK];
     double pri = ((double)sec[S_PRI] / K_FREF) * 1e-6;
     double dt = 1 / fs;
+#if ORC_MUTATION == 8
+    for (int i = 0; i < n; i++) out[i] = (swst + K_LATCH) + ((swst + (rank * pri) + ((double)i * dt)) / 2);
+    return;
+#endif
     for (int i = 0; i < n; i++) out[i] = (swst + K_LATCH) + ((swst + (rank * pri) - ((double)i * dt)) / 2);
 }
 
@@ -766,6 +796,9 @@ void orc_rcmc_mf(double* data, int L, int n, int64_t ld, double prf, double dr, 
                  int apply_rcmc) {
     double* fa = (double*)malloc(sizeof(double) * (size_t)L);
     orc_linspace(-prf / 2, prf / 2, L, fa);
+#if ORC_MUTATION == 7
+    for (int i = 0; i < L; i++) fa[i] = -prf / 2 + i * (prf / L);
+#endif
     const double lam = K_WAVELENGTH;
 #pragma omp parallel
     {
@@ -776,10 +809,20 @@ void orc_rcmc_mf(double* data, int L, int n, int64_t ld, double prf, double dr, 
             memcpy(row, out, sizeof(double) * 2 * (size_t)n);
             for (int j = 0; j < n; j++) {
                 double f = fa[i] + fdc[j];
+#if ORC_MUTATION == 6
+                f = fa[i];
+#endif
+#if ORC_MUTATION == 2
+                double D = sqrt(1 - ((lam * lam) * (f * f)) / (2 * v[j] * v[j]));
+#else
                 double D = sqrt(1 - ((lam * lam) * (f * f)) / (4 * v[j] * v[j]));
+#endif
                 double xr, xi;
                 if (apply_rcmc) {
                     double shift = (R[j] / D) - R[j];
+#if ORC_MUTATION == 5
+                    shift = -shift;
+#endif
                     double t = (double)j + shift / dr;
                     double o[2];
                     orc_sinc_interpolate(row, n, t, 8, o);
@@ -791,10 +834,19 @@ void orc_rcmc_mf(double* data, int L, int n, int64_t ld, double prf, double dr, 
                 }
                 double q = ref_range / R[j];
                 double amp = sqrt(pow(q, 3.0));
+#if ORC_MUTATION == 3
+                amp = 1.0;
+#endif
                 xr = xr / amp;
                 xi = xi / amp;
                 double ph1 = (((4.0 * K_PI) * R[j]) * D) / lam;
                 double ph2 = ((2.0 * K_PI) * (fa[i] + fdc[j])) * tc[j];
+#if ORC_MUTATION == 1
+                ph1 = -ph1;
+#endif
+#if ORC_MUTATION == 4
+                ph2 = -ph2;
+#endif
                 double c1 = cos(ph1), s1 = sin(ph1), c2 = cos(ph2), s2 = sin(ph2);
                 double hr = c1 * c2 - s1 * s2, hi = c1 * s2 + s1 * c2;
                 out[2 * j] = xr * hr - xi * hi;

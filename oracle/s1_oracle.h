@@ -53,6 +53,9 @@ void orc_packet_params(const int64_t* secondary46, double* out);
 void orc_slant_ranges(const int64_t* secondary46, int num_ranges, double* out);
 void orc_timing_corrections(const int64_t* secondary46, double* out);
 void orc_linspace(double start, double end, int size, double* out);
+/* OpenMP threads for the oracle's parallel loops (timing the CPU baseline
+ * at a stated thread count); returns the count in effect. */
+int orc_set_threads(int n);
 
 /* ---- FFT with FFTW's conventions (unnormalised both ways) --------------- */
 void orc_fft(double* x, int n, int inverse);

@@ -65,6 +65,7 @@ _SIGS = [
     ("s1g_synchronize", C.c_int, [C.c_void_p]),
     ("s1g_decode_fdbaq", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                    C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
+    ("s1g_check_status", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     ("s1g_fdbaq_symbol", C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("s1g_parse_packet_header", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(PacketInfo)]),
     ("s1g_slant_ranges", C.c_int, [C.POINTER(PacketInfo), C.c_int, C.c_void_p]),
@@ -269,6 +270,13 @@ class Context:
 
     def synchronize(self):
         self._check(lib().s1g_synchronize(self.h), "synchronize")
+
+    def check_status(self, status):
+        """Wait for the stream and raise S1GError naming the first failed
+        packet of a device status array (s1g_check_status)."""
+        first = C.c_int(-1)
+        self._bind_stream()
+        self._check(lib().s1g_check_status(self.h, _ptr(status), status.numel(), C.byref(first)), "check_status")
 
     # ---------------------------------------------------------------- decode
     def decode(self, user_data, ud_offsets, ud_lengths, num_quads, max_bytes, max_quads, prec=S1G_F32,

@@ -93,6 +93,20 @@ s1g_ctx* ctx() {
     return t_ctx;
 }
 
+// Throws runtime_error naming the first packet whose status[] entry (device
+// array, one per packet) is not S1G_PKT_OK; waits for the stream first.
+void throw_failed_packets(const PACKET_VEC_1D& packets, const int32_t* d_status) {
+    int first = -1;
+    const int rc = s1g_check_status(ctx(), d_status, (int)packets.size(), &first);
+    if (rc == S1G_EDECODE && first >= 0) {
+        std::vector<int32_t> st(packets.size());
+        download(st.data(), d_status, sizeof(int32_t) * st.size());
+        throw std::runtime_error("packet " + std::to_string(packets[(size_t)first].get_packet_index()) + ": " +
+                                 kDecodeErrors[std::min(std::max(st[(size_t)first], 0), 5)]);
+    }
+    check(rc, "decode status");
+}
+
 // Rows x cols DFT along `axis` on a dense device array, FP64, in place.
 void dft_dev(cdouble* d, int rows, int cols, int64_t ld, int axis, bool inverse) {
     check(s1g_dft(ctx(), S1G_F64, rows, cols, d, ld, axis, inverse ? 1 : 0), "dft");
@@ -209,7 +223,7 @@ void decode_packets_in_place(PACKET_VEC_1D& packets) {
     if (n == 0) return;
     std::vector<uint64_t> offs(n);
     std::vector<uint32_t> lens(n);
-    std::vector<int32_t> nq(n), status(n);
+    std::vector<int32_t> nq(n);
     size_t total = 0;
     int max_bytes = 0, max_quads = 0;
     for (int i = 0; i < n; ++i) {
@@ -234,12 +248,8 @@ void decode_packets_in_place(PACKET_VEC_1D& packets) {
     const int rc = s1g_decode_fdbaq(ctx(), d_blob.as<uint8_t>(), d_offs.as<uint64_t>(), d_lens.as<uint32_t>(),
                                     d_nq.as<int32_t>(), n, max_bytes, max_quads, S1G_F64, d_out.p, ld,
                                     d_st.as<int32_t>());
-    if (rc != S1G_OK && rc != S1G_EDECODE) check(rc, "decode");
-    download(status.data(), d_st.p, 4 * (size_t)n);
-    for (int i = 0; i < n; ++i)
-        if (status[i] != S1G_PKT_OK)
-            throw std::runtime_error("packet " + std::to_string(packets[i].get_packet_index()) + ": " +
-                                     kDecodeErrors[std::min(std::max(status[i], 0), 5)]);
+    check(rc, "decode");
+    throw_failed_packets(packets, d_st.as<int32_t>());
     std::vector<cdouble> host((size_t)n * ld);
     download(host.data(), d_out.p, sizeof(cdouble) * host.size());
     for (int i = 0; i < n; ++i)
@@ -496,14 +506,10 @@ CF_VEC_2D run_burst(PACKET_VEC_1D& packets, bool azimuth, bool do_azimuth_fft, c
         if (rc == S1G_OK && do_azimuth_fft) rc = s1g_azimuth_fft(c, prec, L, N, d_out.p, N, 0);
         if (rc == S1G_OK) rc = s1g_synchronize(c);
     }
-    if (rc == S1G_EDECODE) {
-        std::vector<int32_t> st((size_t)L);
-        download(st.data(), d_st.p, 4 * (size_t)L);
-        for (int i = 0; i < L; ++i)
-            if (st[i]) throw std::runtime_error("packet " + std::to_string(packets[i].get_packet_index()) + ": " +
-                                                kDecodeErrors[std::min(std::max(st[i], 0), 5)]);
-    }
     check(rc, azimuth ? "azimuth compression" : "range compression");
+    // the reference throws out of the decode (packet.cpp:795-798,
+    // decoding_utils.cpp:69-72,107-119); the chain reports per packet
+    throw_failed_packets(packets, d_st.as<int32_t>());
     CF_VEC_2D out((size_t)L, CF_VEC_1D((size_t)N));
     if (prec == S1G_F64) {
         download_rows(out, N, d_out.as<cdouble>());

@@ -693,7 +693,11 @@ __device__ __forceinline__ void sinc_weights(float fr, float* w) {
         const float v = A * num * __builtin_amdgcn_rcpf(fr - (float)k);
         w[k + kTaps] = (k & 1) ? -v : v;
     }
+    // the two exact sample positions: 0 * inf above.  fr = 1 is reached when
+    // an FP64 offset within 2^-25 below an integer rounds up to 1.0f; its
+    // weights select sample n0 + 1, the reference's result at t = n0 + 1.
     if (fr == 0.0f) w[kTaps] = 1.0f;
+    if (fr == 1.0f) w[kTaps + 1] = 1.0f;
 }
 
 // FP64: the reference's per-tap expression sin(PI x)/(PI x) * window with

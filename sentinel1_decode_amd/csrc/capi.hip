@@ -55,8 +55,18 @@ struct s1g_ctx {
     size_t af_n = 0;
     void* af_img = nullptr;
     size_t af_img_bytes = 0;
-    std::map<std::string, hipGraphExec_t> graphs;
-    std::vector<hipGraph_t> graph_objs;
+    // Captured batch graphs bake the workspace pointers (ws, cols,
+    // eccm_scratch) into their kernel arguments: each remembers the
+    // workspace generation it was captured under, and ensure() moves the
+    // generation on whenever it reallocates, so a stale graph is captured
+    // again instead of replayed into freed memory.
+    struct CachedGraph {
+        hipGraphExec_t exec = nullptr;
+        hipGraph_t graph = nullptr;
+        uint64_t gen = 0;
+    };
+    std::map<std::string, CachedGraph> graphs;
+    uint64_t ws_gen = 0;
     // stage timers
     bool profiling = false;
     bool capturing = false;
@@ -217,6 +227,9 @@ int get_fft(s1g_ctx* c, int n, const DevFft** out) {
 
 int ensure(s1g_ctx* c, void** p, size_t* have, size_t need) {
     if (*have >= need) return 0;
+    // everything a captured chain uses is sized before the capture starts
+    if (c->capturing) return fail(c, S1G_EHIP, "workspace reallocation during graph capture%s");
+    ++c->ws_gen;
     if (*p) hipFree(*p);
     *p = nullptr;
     *have = 0;
@@ -449,6 +462,30 @@ int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_
 
 }  // namespace
 
+// Context internals for entry points defined in other files.
+namespace s1g {
+hipStream_t ctx_stream(const s1g_ctx* c) { return c->stream; }
+int ctx_device(const s1g_ctx* c) { return c->device; }
+int ctx_error(s1g_ctx* c, int code, const char* msg) {
+    c->err = msg;
+    return code;
+}
+int ctx_rcmc_variant(s1g_ctx* c, int prec, int n_lines, int n_samples, int apply_rcmc, int fallback) {
+    const int nvar = rcmc_variants(prec);
+    int v = fallback;
+    if (c->force[S1G_STAGE_RCMC_MF] >= 0) {
+        v = c->force[S1G_STAGE_RCMC_MF];
+    } else {
+        auto it = c->tuned.find(TuneKey{S1G_STAGE_RCMC_MF, prec, n_lines, n_samples, apply_rcmc});
+        if (it != c->tuned.end()) v = it->second;
+    }
+    v = std::max(0, std::min(v, nvar - 1));
+    c->last_var[S1G_STAGE_RCMC_MF] = v;
+    c->last_nvar[S1G_STAGE_RCMC_MF] = nvar;
+    return v;
+}
+}  // namespace s1g
+
 // ===================================================================== ABI
 extern "C" {
 
@@ -481,8 +518,10 @@ int s1g_destroy(s1g_ctx* c) {
     for (auto e : c->event_pool) hipEventDestroy(e);
     if (c->side) hipStreamSynchronize(c->side);
     for (auto e : c->sync_events) hipEventDestroy(e);
-    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
-    for (auto g : c->graph_objs) hipGraphDestroy(g);
+    for (auto& kv : c->graphs) {
+        hipGraphExecDestroy(kv.second.exec);
+        hipGraphDestroy(kv.second.graph);
+    }
     for (auto& kv : c->ffts) {
         hipFree(kv.second.tw64);
         hipFree(kv.second.tw32);
@@ -703,6 +742,29 @@ int s1g_decode_fdbaq(s1g_ctx* c, const uint8_t* ud, const uint64_t* offs, const 
                                  status);
     if (rc == -1) return fail(c, S1G_EUNSUPPORTED, "packet larger than the on-chip decode buffer%s");
     if (rc) return hip_fail(c, (hipError_t)rc, "fdbaq decode");
+    return S1G_OK;
+}
+
+int s1g_check_status(s1g_ctx* c, const int32_t* status, int n_packets, int* first_failed) {
+    if (!c) return S1G_EARG;
+    if (first_failed) *first_failed = -1;
+    if (n_packets < 0 || (n_packets && !status)) return fail(c, S1G_EARG, "bad status array%s");
+    if (n_packets == 0) return S1G_OK;
+    std::vector<int32_t> h((size_t)n_packets);
+    HIPCHK(c, hipMemcpyAsync(h.data(), status, sizeof(int32_t) * (size_t)n_packets, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    static const char* const what[6] = {"ok", "BRC value above 4", "read past the end of the user data",
+                                        "Huffman code not found", "magnitude code above the BRC flag",
+                                        "packet above the stated maxima"};
+    for (int i = 0; i < n_packets; ++i) {
+        if (h[i] == S1G_PKT_OK) continue;
+        if (first_failed) *first_failed = i;
+        char buf[160];
+        snprintf(buf, sizeof buf, "packet %d: %s (status %d)", i, what[std::min(std::max(h[i], 0), 5)], h[i]);
+        c->err = buf;
+        return S1G_EDECODE;
+    }
     return S1G_OK;
 }
 
@@ -1060,6 +1122,14 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
     key.append((const char*)&prec, sizeof prec);
     key.append((const char*)c->force, sizeof c->force);
     auto it = c->graphs.find(key);
+    if (it != c->graphs.end() && it->second.gen != c->ws_gen) {
+        // captured before a workspace reallocation: its pointers are stale
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        hipGraphExecDestroy(it->second.exec);
+        hipGraphDestroy(it->second.graph);
+        c->graphs.erase(it);
+        it = c->graphs.end();
+    }
     if (it == c->graphs.end()) {
         // one eager pass first: it picks (and caches) each stage's launch
         // variant and builds everything the captured launches use
@@ -1091,10 +1161,13 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
             hipGraphDestroy(g);
             return hip_fail(c, e, "hipGraphInstantiate");
         }
-        c->graph_objs.push_back(g);
-        it = c->graphs.emplace(key, ge).first;
+        s1g_ctx::CachedGraph cg;
+        cg.exec = ge;
+        cg.graph = g;
+        cg.gen = c->ws_gen;
+        it = c->graphs.emplace(key, cg).first;
     }
-    HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+    HIPCHK(c, hipGraphLaunch(it->second.exec, c->stream));
     return S1G_OK;
 }
 

@@ -418,7 +418,7 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
                                                            const uint32_t* __restrict__ lens,
                                                            const int32_t* __restrict__ nqs, int n_packets,
                                                            int max_words, int lds_words, int max_blocks,
-                                                           T* __restrict__ out, int64_t ld_out,
+                                                           int max_quads, T* __restrict__ out, int64_t ld_out,
                                                            int32_t* __restrict__ status) {
     extern __shared__ __align__(16) uint32_t smem[];  // base 16-aligned after the static words
     __shared__ int s_st, s_st_qo;  // status after IE..QE; status of the QO walk
@@ -446,8 +446,10 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     const int nb = nq > 0 ? (nq + 127) / 128 : 0;
     T* orow = out + (int64_t)2 * ld_out * p;
     int st = S1G_PKT_OK;
-    if (nq < 0 || nb > max_blocks || nb > kMaxBlocks || nbytes_u > (uint32_t)kMaxPacketBytes ||
-        (nbytes + 3) / 4 > max_words)
+    // a packet above the stated maxima would write past its row (ld_out is
+    // only checked against 2 * max_quads) or overflow the LDS tables
+    if (nq < 0 || nq > max_quads || nb > max_blocks || nb > kMaxBlocks ||
+        nbytes_u > (uint32_t)kMaxPacketBytes || (nbytes + 3) / 4 > max_words)
         st = S1G_PKT_ERR_ARG;
 
     PacketWords<ALL> words;
@@ -612,7 +614,7 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     const bool all = max_words + kPadWords <= kLdsWords;
 #define S1G_LAUNCH(T, A)                                                                                       \
     hipLaunchKernelGGL((fdbaq_decode_kernel<T, A>), grid, block, lds, stream, ud, offs, lens, nqs, n_packets, \
-                       max_words, lds_words, max_blocks, (T*)out, ld_out, status)
+                       max_words, lds_words, max_blocks, max_quads, (T*)out, ld_out, status)
     if (prec == S1G_F32) {
         if (all) S1G_LAUNCH(float, true);
         else S1G_LAUNCH(float, false);

@@ -54,13 +54,23 @@ struct AzUniform {
 };
 
 // Bytes of the column table of n columns: AzColumn[n], AzUniform, float amplitude[n].
-inline size_t az_table_bytes(int n) {
+constexpr size_t az_table_bytes(int n) {
     return ((sizeof(AzColumn) * (size_t)n + sizeof(AzUniform) + 15) & ~(size_t)15) + sizeof(float) * (size_t)n;
 }
 inline AzUniform* az_uniform_of(AzColumn* cols, int n) { return (AzUniform*)(cols + n); }
 inline float* az_amp_of(AzColumn* cols, int n) {
     return (float*)((char*)cols + ((sizeof(AzColumn) * (size_t)n + sizeof(AzUniform) + 15) & ~(size_t)15));
 }
+
+// Context internals (capi.hip) for entry points defined in other files.
+hipStream_t ctx_stream(const s1g_ctx* c);
+int ctx_device(const s1g_ctx* c);
+// Sets the context's error message and returns code.
+int ctx_error(s1g_ctx* c, int code, const char* msg);
+// RCMC launch variant for a shape without timing any: the forced variant,
+// else the tuner's cached choice, else `fallback`; recorded as the stage's
+// last variant.
+int ctx_rcmc_variant(s1g_ctx* c, int prec, int n_lines, int n_samples, int apply_rcmc, int fallback);
 
 int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
                         const int32_t* nqs, int n_packets, int max_bytes, int max_quads, void* out, int64_t ld_out,

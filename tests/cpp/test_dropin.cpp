@@ -267,6 +267,24 @@ static void packets_and_bursts(const std::string& dir) {
         }
         report("truncated user data throws", threw, 0);
     }
+    {   // the same inside a burst: the reference's decode throws out of
+        // _range_compress (s1_decoder.cpp:462-468), so the burst calls throw
+        PACKET_VEC_1D bad = packets;
+        const_cast<UINT8_VEC_1D&>(bad[L / 2].get_user_data()).resize(16);
+        bool threw_az = false, threw_rc = false;
+        try {
+            (void)azimuth_compress_burst(bad, {geo[0]}, {geo[1]});
+        } catch (const std::runtime_error& e) {
+            threw_az = std::string(e.what()).find("packet " + std::to_string(L / 2)) != std::string::npos;
+        }
+        try {
+            (void)range_compress_burst(bad);
+        } catch (const std::runtime_error& e) {
+            threw_rc = std::string(e.what()).find("packet " + std::to_string(L / 2)) != std::string::npos;
+        }
+        report("truncated packet in azimuth_compress_burst throws", threw_az, 0);
+        report("truncated packet in range_compress_burst throws", threw_rc, 0);
+    }
 }
 
 int main(int argc, char** argv) {
