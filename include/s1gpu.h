@@ -196,6 +196,22 @@ typedef struct {
 int s1g_rcmc_matched_filter(s1g_ctx* ctx, const s1g_azimuth_desc* desc, int prec, const void* in,
                             int64_t ld_in, void* out, int64_t ld_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same stage and the same kernels as
+ * s1g_rcmc_matched_filter, but its per-column geometry table lives in a
+ * fixed device array of the library (one per GPU, about 5.5 MB, held until
+ * the process ends) instead of context memory, and the call makes no device
+ * allocation, frees nothing and times no launch variants (it uses a forced
+ * variant, the one the tuner already chose for the shape, or the tuner's
+ * usual pick).  n_samples at most S1G_NOALLOC_MAX_SAMPLES (S1G_EARG above
+ * it).  Every call on a GPU shares that GPU's table, which each call
+ * rewrites in full: calls that overlap on one GPU (other streams or other
+ * contexts) race. */
+#define S1G_NOALLOC_MAX_SAMPLES 65536
+int s1g_rcmc_matched_filter_noalloc(s1g_ctx* ctx, const s1g_azimuth_desc* desc, int prec, const void* in,
+                                    int64_t ld_in, void* out, int64_t ld_out);
+
 /* ---------------------------------------------------------------- ECCM */
 /* RFI detection and notch, in place (eccm, signal_processing.cpp:913-1004,
  * as called from s1_decoder.cpp:470-474 with fft_size 128, stride 64).

@@ -76,6 +76,8 @@ _SIGS = [
     ("s1g_azimuth_fft", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_int]),
     ("s1g_rcmc_matched_filter", C.c_int, [C.c_void_p, C.POINTER(AzimuthDesc), C.c_int, C.c_void_p, C.c_int64,
                                           C.c_void_p, C.c_int64]),
+    ("s1g_rcmc_matched_filter_noalloc", C.c_int, [C.c_void_p, C.POINTER(AzimuthDesc), C.c_int, C.c_void_p,
+                                                  C.c_int64, C.c_void_p, C.c_int64]),
     ("s1g_eccm", C.c_int, [C.c_void_p, C.POINTER(EccmDesc), C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
     ("s1g_focus_burst", C.c_int, [C.c_void_p, C.POINTER(BurstDesc), C.c_int]),
     ("s1g_focus_batch", C.c_int, [C.c_void_p, C.POINTER(BurstDesc), C.c_int, C.c_int, C.c_int]),
