@@ -1,10 +1,15 @@
 /*
- * image_formation.h -- drop-in get_reference_function and pulse_compression
- * (image_formation.h:14-21 of the reference), on the GPU (FP64).
+ * image_formation.h -- drop-in image-formation functions of the reference's
+ * image_formation.h (lines 14-69) used on the L0 -> SLC path, on the GPU
+ * (FP64).  The TOPS resampling (azimuth_frequency_ufr, azimuth_time_ufr) is
+ * not provided.
  */
 #pragma once
 
+#include <utility>
+
 #include "misc_types.h"
+#include "packet.h"
 
 /* signal (a line's range spectrum) times reference, elementwise over the
  * reference's length, truncated to the signal's length
@@ -14,3 +19,23 @@ CF_VEC_1D pulse_compression(const CF_VEC_1D& signal, const CF_VEC_1D& reference)
 
 /* conj(DFT(replica_chirp)) / E, E = sum |c|^4 / n (image_formation.cpp:4-25). */
 CF_VEC_1D get_reference_function(const CF_VEC_1D& replica_chirp);
+
+/* Stripmap packets split where the line length (2 * num_quads) changes,
+ * with the longest line length (image_formation.cpp:46-86). */
+std::pair<PACKET_VEC_2D, int> get_azimuth_blocks(PACKET_VEC_1D& packets);
+
+/* Effective velocity per slant range from the platform's ECEF position and
+ * speed over the WGS84 ellipsoid (image_formation.cpp:391-419). */
+D_VEC_1D get_effective_velocities(const D_VEC_1D& position, const double& velocity, const D_VEC_1D& slant_ranges);
+
+/* Multiplies the range line by the secondary range compression filter
+ * exp(-i PI f_r^2 / K_src) and returns the range-cell-migration factors
+ * D = sqrt(1 - lambda^2 (f_a + f_dc)^2 / (4 v^2)) (image_formation.cpp:422-455). */
+D_VEC_1D apply_secondary_range_compression(CF_VEC_1D& range_line, const D_VEC_1D& effective_velocities,
+                                           const D_VEC_1D& slant_ranges, const D_VEC_1D& range_freqs,
+                                           const D_VEC_1D& doppler_centroids, const double& az_freq);
+
+/* Multiplies the range line by exp(-4 i PI f_r (R / D - R) / c)
+ * (image_formation.cpp:458-476). */
+void apply_range_cell_migration_correction(CF_VEC_1D& range_line, const D_VEC_1D& slant_ranges,
+                                           const D_VEC_1D& range_freqs, const D_VEC_1D& rcmc_factors);

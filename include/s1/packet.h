@@ -2,8 +2,9 @@
  * packet.h -- drop-in L0Packet (packet.h:63-215 of the reference) for the
  * Level-0 -> SLC path: header fields and the physical quantities derived
  * from them, the replica chirp, and the FDBAQ-decoded signal computed on
- * the GPU (s1g_decode_fdbaq).  Members outside the path (printing, mode
- * strings, state-vector words) are not provided.
+ * the GPU (s1g_decode_fdbaq), and reading a Level-0 file into packets and
+ * bursts.  Members outside the path (printing, mode strings) are not
+ * provided.
  *
  * Errors are C++ exceptions as in the reference: a packet whose user data
  * fails to decode throws std::runtime_error from get_signal() and
@@ -50,6 +51,18 @@ class L0Packet {
     [[nodiscard]] double get_rx_gain() const { return _info.rx_gain_db; }
     [[nodiscard]] double get_range_sample_rate() const { return _info.fs_mhz; }
 
+    /* coarse_time + 1 / fine_time, as the reference computes it
+     * (packet.cpp:56-63). */
+    [[nodiscard]] double get_time() const;
+    /* The swath's name from swath_number (SWATH_NUM_TO_STRING, structs.h:489-559). */
+    [[nodiscard]] std::string get_swath() const;
+    /* 'V' / 'H' from rx_channel_id 0 / 1, and from the polarisation code
+     * 0-3 / 4-7 (packet.cpp:104-122); other codes throw std::runtime_error. */
+    [[nodiscard]] char get_rx_polarization() const;
+    [[nodiscard]] char get_tx_polarization() const;
+    /* Two-way times of the range samples: linspace from rank * PRI + SWST +
+     * the suppressed-data time over 2 * num_quads / fs (packet.cpp:304-318). */
+    [[nodiscard]] D_VEC_1D get_slant_range_times(int num_ranges = 0) const;
     [[nodiscard]] D_VEC_1D get_slant_ranges(int num_ranges = 0) const;
     [[nodiscard]] D_VEC_1D get_timing_corrections() const;
     [[nodiscard]] CF_VEC_1D get_replica_chirp() const;
@@ -76,6 +89,21 @@ class L0Packet {
 
 typedef L0Packet::PACKET_VEC_1D PACKET_VEC_1D;
 typedef L0Packet::PACKET_VEC_2D PACKET_VEC_2D;
+
+/* The packets of a Level-0 file (get_packets, packet.cpp:880-905): all of
+ * them when num_packets is 0.  The file is read whole and indexed with
+ * s1g_index_level0; a packet that cannot be constructed is reported and
+ * skipped, as the reference skips it. */
+PACKET_VEC_1D get_packets(const std::string& filename, int num_packets = 0);
+/* The packets of one swath (get_packets_in_swath, packet.cpp:917-951);
+ * std::runtime_error when there are none. */
+PACKET_VEC_1D get_packets_in_swath(const std::string& filename, const std::string& swath);
+/* The swath's packets split into bursts (get_packets_in_bursts,
+ * packet.cpp:1026-1076): a burst ends where azimuth_beam_address decreases
+ * or pri_count jumps; bursts of fewer than 1000 packets are dropped except
+ * the last; calibration swaths stay whole (get_cal_packets). */
+PACKET_VEC_2D get_packets_in_bursts(PACKET_VEC_1D& packets, const std::string& swath, bool get_cal_packets = false);
+PACKET_VEC_2D get_packets_in_bursts(const std::string& filename, const std::string& swath);
 
 /* decode_packets_in_place (packet.h:228): every packet's signal, decoded in
  * one GPU launch.  decode_packets (packet.h:238): the same on a copy. */

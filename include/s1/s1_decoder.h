@@ -1,14 +1,25 @@
 /*
- * s1_decoder.h -- the burst-level work of S1_Decoder for one burst's
- * packets (the bodies of S1_Decoder::_range_compress and _azimuth_compress,
- * s1_decoder.cpp:448-701, on their non-TOPS formulas), on the GPU.  File
- * reading, swath/burst bookkeeping and orbit processing of S1_Decoder are
- * outside the path: the caller passes the burst's packets and the effective
- * velocity / Doppler centroid per range sample (one value broadcasts).
+ * s1_decoder.h -- drop-in S1_Decoder (s1_decoder.h:13-77 of the reference):
+ * a Level-0 file's packets split into swaths and bursts, its state vectors,
+ * and the decoded, range-compressed and focused bursts and swaths, computed
+ * on the GPU.  Also the burst-level bodies of S1_Decoder::_range_compress
+ * and _azimuth_compress (s1_decoder.cpp:448-701) as free functions over one
+ * burst's packets.
+ *
+ * Focusing uses the range-Doppler chain (the reference's non-TOPS formulas:
+ * decode, ECCM, range compression, azimuth FFT, RCMC + matched filter,
+ * azimuth IFFT).  For IW bursts the reference runs its TOPS variant
+ * (Doppler-centroid estimation and azimuth resampling), which is not
+ * provided: here the Doppler centroid is zero, as in its non-TOPS branch,
+ * and the effective velocity per range sample comes from the state vectors
+ * interpolated at the burst's middle row.
  */
 #pragma once
 
+#include <set>
+
 #include "packet.h"
+#include "state_vectors.h"
 
 struct S1_ECCM_THRESHOLDS {
     double detection;
@@ -25,3 +36,47 @@ CF_VEC_2D range_compress_burst(PACKET_VEC_1D& packets, const bool& do_azimuth_ff
  * (S1_Decoder::_azimuth_compress, s1_decoder.cpp:511-701). */
 CF_VEC_2D azimuth_compress_burst(PACKET_VEC_1D& packets, const D_VEC_1D& effective_velocities,
                                  const D_VEC_1D& doppler_centroids, const S1_ECCM_THRESHOLDS* eccm = nullptr);
+
+class S1_Decoder {
+   public:
+    S1_Decoder() {}
+    explicit S1_Decoder(const std::string& filename);
+
+    /* ECCM before range compression with the given thresholds
+     * (s1_decoder.cpp:37-44); apply_eccm() without thresholds uses the
+     * reference's defaults of zero. */
+    void apply_eccm() { _use_eccm = true; }
+    void apply_eccm(const int& detection_threshold, const int& mitigation_threshold);
+
+    [[nodiscard]] STATE_VECTORS get_state_vectors() { return _state_vectors; }
+
+    [[nodiscard]] CF_VEC_2D get_burst(const std::string& swath, const int& burst);
+    [[nodiscard]] CF_VEC_2D get_swath(const std::string& swath);
+    [[nodiscard]] CF_VEC_2D get_range_compressed_burst(const std::string& swath, const int& burst,
+                                                       const bool& range_doppler = false);
+    [[nodiscard]] CF_VEC_2D get_range_compressed_swath(const std::string& swath, const bool& range_doppler = false);
+    [[nodiscard]] CF_VEC_2D get_azimuth_compressed_burst(const std::string& swath, const int& burst);
+    [[nodiscard]] CF_VEC_2D get_azimuth_compressed_swath(const std::string& swath);
+
+    /* Not in the reference's interface: the swaths present with their packet
+     * counts, the bursts of a swath, and the effective velocities the
+     * focusing of a burst uses. */
+    [[nodiscard]] const std::unordered_map<std::string, int>& swath_counts() const { return _swath_counts; }
+    [[nodiscard]] const PACKET_VEC_2D& bursts(const std::string& swath);
+    [[nodiscard]] D_VEC_1D burst_effective_velocities(PACKET_VEC_1D& packets);
+
+   private:
+    std::string _filename;
+    bool _use_eccm = false;
+    int _eccm_detection_threshold = 0;
+    int _eccm_mitigation_threshold = 0;
+    PACKET_VEC_1D _flat_packets;
+    std::unordered_map<std::string, int> _swath_counts;
+    std::unordered_map<std::string, PACKET_VEC_2D> _echo_packets;
+    std::unordered_map<std::string, PACKET_VEC_2D> _cal_packets;
+    STATE_VECTORS _state_vectors;
+
+    void _validate_request(const std::string& swath, const int& burst = -1);
+    CF_VEC_2D _range_compress(PACKET_VEC_1D& packets, const bool& do_azimuth_fft = false);
+    CF_VEC_2D _azimuth_compress(PACKET_VEC_1D& packets);
+};

@@ -2,8 +2,9 @@
  * signal_processing.h -- drop-in building blocks of the reference's
  * signal_processing.h used on the L0 -> SLC path, computed on the GPU
  * through the C ABI (FP64).  Signatures follow signal_processing.h:15-180.
- * get_fftw_plans / destroy_fftw_plans expose FFTW types and have no
- * counterpart: plans live in the s1g context.
+ * get_fftw_plans / destroy_fftw_plans return opaque plan handles
+ * (s1g_plan) in place of FFTW's; code that includes these headers must not
+ * also include fftw3.h.
  */
 #pragma once
 
@@ -37,3 +38,18 @@ void fftshift_in_place(CF_VEC_2D& signals);
  * is unused, as in the reference. */
 void eccm(CF_VEC_2D& signals, const int& fft_size, const int& stride, const double& detection_threshold,
           const double& mitigation_threshold, const char& rx_pol = 'V');
+
+/* Per-row FFT plans (signal_processing.cpp:684-710): plan k binds row k of
+ * `signals` in place, unnormalised in both directions, like the FFTW plans
+ * it replaces; fftw_execute(plan) transforms the bound row on the GPU. */
+namespace s1 {
+struct fft_plan_rec;
+}
+typedef s1::fft_plan_rec* fftw_plan;
+#ifndef FFTW_FORWARD
+#define FFTW_FORWARD (-1)
+#define FFTW_BACKWARD (+1)
+#endif
+std::vector<fftw_plan> get_fftw_plans(CF_VEC_2D& signals, const int& direction = FFTW_FORWARD);
+void destroy_fftw_plans(std::vector<fftw_plan>& plans);
+void fftw_execute(const fftw_plan plan);

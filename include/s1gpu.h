@@ -131,6 +131,95 @@ int s1g_parse_packet_header(const uint8_t* packet, int64_t nbytes, s1g_packet_in
 int s1g_slant_ranges(const s1g_packet_info* info, int num_ranges, double* out);
 int s1g_timing_corrections(const s1g_packet_info* info, double* out);
 
+/* ------------------------------------------ Level-0 files: framing */
+/* A Level-0 file is a sequence of space packets: a 6-byte primary header
+ * whose packet_data_length field (bytes 4-5) gives the packet's length,
+ * 6 + packet_data_length + 1 bytes, a 62-byte secondary header, and the
+ * user data field (get_next_packet / get_packets, packet.cpp:850-905). */
+#define S1G_NFIELDS 54 /* 8 primary + 46 secondary header fields */
+
+/* Host function.  Walks the packets of a Level-0 byte stream (host memory)
+ * and writes the byte offset of each packet's first header byte to
+ * offsets[0 .. min(count, cap)); *n_packets = count.  Stops at the end of
+ * the stream; a last packet that runs past it is dropped, as the
+ * reference's reader drops it. */
+int s1g_index_level0(const uint8_t* stream, int64_t nbytes, uint64_t* offsets, int64_t cap, int64_t* n_packets);
+
+/* Parses the 6 + 62 header bytes of n_packets packets on the GPU (one
+ * wavefront per packet, one lane per field; parse_packet_header,
+ * packet.cpp:829-846).  stream, offsets: device pointers (offsets[k] =
+ * first header byte of packet k).  Outputs (device, any may be NULL):
+ *   fields[k * S1G_NFIELDS + f]  header field f (primary fields 0-7, then
+ *                                secondary fields in structs.h:68-115 order)
+ *   ud_offsets[k], ud_lengths[k], num_quads[k]  the user-data offset
+ *                                (offsets[k] + 68), length (packet_data_length
+ *                                + 1 - 62) and num_quadratures: the arrays
+ *                                s1g_decode_fdbaq takes. */
+int s1g_parse_headers(s1g_ctx* ctx, const uint8_t* stream, const uint64_t* offsets, int n_packets,
+                      int64_t* fields, uint64_t* ud_offsets, uint32_t* ud_lengths, int32_t* num_quads);
+
+/* Host function.  The bursts of one swath (get_packets_in_bursts,
+ * packet.cpp:1026-1076) over the packets' header fields (host, as
+ * s1g_parse_headers writes them): among packets whose swath_number equals
+ * `swath_number`, a burst ends where azimuth_beam_address decreases or
+ * pri_count jumps by more than one; a burst that ends with fewer than 1000
+ * packets is dropped, the last one is kept whatever its size; with cal != 0
+ * all the swath's packets form one burst.  burst_of_packet[k] = the burst
+ * index of packet k, or -1 (other swaths, dropped bursts); *n_bursts = the
+ * number of bursts. */
+int s1g_segment_bursts(const int64_t* fields, int n_packets, int swath_number, int cal, int32_t* burst_of_packet,
+                       int* n_bursts);
+
+/* ------------------------------------ orbit geometry (ancillary data) */
+/* One state vector: the time stamp (seconds), ECEF position (m) and
+ * velocity (m/s), and the raw attitude words (quaternion q0..q3 and angular
+ * rates as transmitted; their conversion to Euler angles is not provided). */
+typedef struct {
+    double time;
+    double position[3];
+    double velocity[3];
+    double quaternion[4];
+    double angular_rate[3];
+} s1g_state_vector;
+
+/* Host function.  The state vectors carried by the packets' sub-commutated
+ * ancillary words (sc_data_word_index / sc_data_word: build_data_word_dicts
+ * and STATE_VECTORS::parse_subcomm_dicts, state_vectors.cpp:19-91): one per
+ * complete cycle of 64 words, kept when its time stamp exceeds every earlier
+ * one.  fields: host [n_packets][S1G_NFIELDS].  Writes at most cap vectors;
+ * *n_out = the number found. */
+int s1g_state_vectors(const int64_t* fields, int n_packets, s1g_state_vector* out, int cap, int* n_out);
+
+/* Host function.  STATE_VECTORS::interpolate (state_vectors.cpp:141-200):
+ * linear interpolation between the vectors bracketing `time` (bisection as
+ * the reference does it), the first / last vector outside their span. */
+int s1g_interpolate_state(const s1g_state_vector* vectors, int n, double time, s1g_state_vector* out);
+
+/* Effective velocities for n_lines platform states and n_samples slant
+ * ranges (get_effective_velocities, image_formation.cpp:391-419):
+ * out[i * ld_out + j] = v_rel of line i at range j, from positions[3 i ..
+ * 3 i + 2] (ECEF, m) and velocity_norms[i] (m/s) over the WGS84 ellipsoid.
+ * Device pointers, FP64. */
+int s1g_effective_velocities(s1g_ctx* ctx, const double* positions, const double* velocity_norms, int n_lines,
+                             const double* slant_ranges, int n_samples, double* out, int64_t ld_out);
+
+/* apply_secondary_range_compression (image_formation.cpp:422-455) on
+ * n_rows range lines: for row i and sample j, with f = az_freqs[i] +
+ * f_dc[j], D = sqrt(1 - lambda^2 f^2 / (4 v^2)) (v = v_rel[i * n_samples +
+ * j]) and K = 2 v^2 f0^3 D^2 / (c R_j f^2),
+ *   data[i][j] *= exp(-i PI range_freqs[j]^2 / K),
+ *   rcmc_factors[i * n_samples + j] = D (may be NULL).
+ * Device pointers; data complex of `prec`, the rest FP64. */
+int s1g_secondary_range_compression(s1g_ctx* ctx, int prec, int n_rows, int n_samples, void* data, int64_t ld,
+                                    const double* v_rel, const double* slant_ranges, const double* range_freqs,
+                                    const double* f_dc, const double* az_freqs, double* rcmc_factors);
+
+/* apply_range_cell_migration_correction (image_formation.cpp:458-476) on
+ * n_rows lines: data[i][j] *= exp(-4 i PI range_freqs[j] (R_j / D_ij - R_j) / c),
+ * D_ij = rcmc_factors[i * n_samples + j].  Device pointers. */
+int s1g_rcmc_phase(s1g_ctx* ctx, int prec, int n_rows, int n_samples, void* data, int64_t ld,
+                   const double* slant_ranges, const double* range_freqs, const double* rcmc_factors);
+
 /* ---------------------------------------------------- range compression */
 /* Replica-chirp parameters of a burst (from its first packet:
  * L0Packet::get_start_frequency / get_tx_ramp_rate / get_pulse_length /
@@ -281,6 +370,17 @@ int s1g_focus_batch(s1g_ctx* ctx, const s1g_burst_desc* descs, int n_bursts, int
  * passes when n factors over radices up to 71, else Bluestein's chirp-z
  * transform through a power-of-two length. */
 int s1g_dft(s1g_ctx* ctx, int prec, int n_rows, int n_cols, void* data, int64_t ld, int axis, int inverse);
+
+/* An FFT plan: the opaque handle that stands in for the per-row FFTW plans
+ * of get_fftw_plans / destroy_fftw_plans (signal_processing.cpp:684-710).
+ * Like those plans it is unnormalised in both directions (inverse = 0:
+ * e^{-2 pi i jk/n}, 1: e^{+...}, no 1/n).  A plan belongs to its context's
+ * stream; s1g_plan_execute transforms n_rows rows of length n of a device
+ * array in place. */
+typedef struct s1g_plan s1g_plan;
+int s1g_plan_create(s1g_ctx* ctx, int prec, int n, int inverse, s1g_plan** plan);
+int s1g_plan_execute(s1g_plan* plan, void* data, int64_t ld, int n_rows);
+int s1g_plan_destroy(s1g_plan* plan);
 
 /* dst[c][r] = src[r][c] (transpose, signal_processing.cpp:463-498); out of place. */
 int s1g_transpose(s1g_ctx* ctx, int prec, int n_rows, int n_cols, const void* src, int64_t ld_src, void* dst,

@@ -96,6 +96,9 @@ def _bind(L):
     L.orc_eccm.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double, _u8]
     L.orc_af_entropy_grad.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, _dp, _dp]
     L.orc_af_entropy_grad.restype = C.c_double
+    L.orc_effective_velocities.argtypes = [_dp, C.c_double, _dp, C.c_int, _dp]
+    L.orc_secondary_range_compression.argtypes = [_dp, C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp]
+    L.orc_rcmc_phase.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
     L.orc_set_threads.argtypes = [C.c_int]
     L.orc_set_threads.restype = C.c_int
     return L
@@ -352,3 +355,159 @@ def af_entropy_grad(rd, phase):
     g = np.zeros(L)
     e = lib().orc_af_entropy_grad(_as_f64(rd), L, N, N, ph, g)
     return e, g
+
+
+# ------------------------------------------------------- Level-0 files
+def index_level0(stream) -> list:
+    """get_next_packet / get_packets (packet.cpp:850-905): packet k starts
+    where packet k-1 ends; its length is 6 + packet_data_length + 1 (primary
+    header bytes 4-5); a packet running past the end of the stream is not
+    returned.  Returns the packets' start offsets."""
+    b = bytes(stream)
+    out, pos = [], 0
+    while len(b) - pos >= 6:
+        n = 6 + ((b[pos + 4] << 8) | b[pos + 5]) + 1
+        if pos + n > len(b):
+            break
+        out.append(pos)
+        pos += n
+    return out
+
+
+SEC = {name: 8 + i for i, name in enumerate([
+    "coarse_time", "fine_time", "sync_marker", "data_take_id", "ecc_number", "na_1", "test_mode",
+    "rx_channel_id", "instrument_configuration_id", "sc_data_word_index", "sc_data_word", "space_packet_count",
+    "pri_count", "error_flag", "na_2", "baq_mode", "baq_block_length", "na_3", "range_decimation", "rx_gain",
+    "tx_ramp_rate_sign", "tx_ramp_rate_mag", "pulse_start_frequency_sign", "pulse_start_frequency_mag",
+    "pulse_length", "na_4", "rank", "pri", "swst", "swl", "ssb_flag", "polarisation", "temperature_compensation",
+    "na_5", "elevation_beam_address", "na_6", "azimuth_beam_address", "calibration_mode", "na_7",
+    "tx_pulse_number", "signal_type", "na_8", "swap", "swath_number", "num_quadratures", "na_9"])}
+
+
+def header_fields(stream, offsets) -> np.ndarray:
+    """[n][54] header fields of the packets at `offsets` (parse_packet_header,
+    packet.cpp:829-846)."""
+    b = np.frombuffer(bytes(stream), np.uint8)
+    out = np.zeros((len(offsets), 54), np.int64)
+    for k, o in enumerate(offsets):
+        p, s_ = parse_headers(b[o: o + 68].tobytes())
+        out[k, :8] = p
+        out[k, 8:] = s_
+    return out
+
+
+def segment_bursts(fields, swath_number, cal=False) -> list:
+    """get_packets_in_bursts (packet.cpp:1026-1076) over the flat packet list:
+    the packet indices of each burst of one swath."""
+    def i32(v):
+        v = int(v) & 0xFFFFFFFF
+        return v - (1 << 32) if v >= 1 << 31 else v
+    bursts, cur = [], []
+    az = pri = prev_az = prev_pri = 0
+    n = len(fields)
+    for i in range(n):
+        f = fields[i]
+        if i32(f[SEC["swath_number"]]) == swath_number:
+            if not cal:
+                az = i32(f[SEC["azimuth_beam_address"]])
+                pri = i32(f[SEC["pri_count"]])
+                if prev_az == 0:
+                    prev_az, prev_pri = az, pri
+                if az < prev_az or pri > prev_pri + 1:
+                    if len(cur) < 1000:
+                        cur = []
+                    else:
+                        bursts.append(cur)
+                        cur = []
+                prev_pri, prev_az = pri, az
+            cur.append(i)
+        if i == n - 1 and len(cur) > 0:
+            bursts.append(cur)
+    return bursts
+
+
+# SUB_COMM_KEY_POS (structs.h:622-689): the quantity of each word index
+_SUBCOMM = (["dummy"] + ["px"] * 4 + ["py"] * 4 + ["pz"] * 4 + ["vx"] * 2 + ["vy"] * 2 + ["vz"] * 2
+            + ["pod"] * 4 + ["q0"] * 2 + ["q1"] * 2 + ["q2"] * 2 + ["q3"] * 2 + ["wx"] * 2 + ["wy"] * 2
+            + ["wz"] * 2 + ["time"] * 4 + ["other"] * 24)
+
+
+def int_to_ieee754(number, is_double):
+    """decoding_utils.cpp:149-170."""
+    msh = 52 if is_double else 23
+    emask = 0x7FF0000000000000 if is_double else 0x7F800000
+    bias = 1023 if is_double else 127
+    ssh = 63 if is_double else 31
+    sign = (number >> ssh) & 1
+    e = ((number & emask) >> msh) - bias
+    total, power = 0.0, -1
+    for i in range(msh):
+        total += ((number >> (msh - i - 1)) & 1) * 2.0 ** power
+        power -= 1
+    return (-1 if sign else 1) * 2.0 ** e * (total + 1.0)
+
+
+def parse_time_stamp(raw):
+    """state_vectors.cpp:5-16, loop as written (only bit 8 of the fine time
+    survives its first step)."""
+    coarse = (raw >> 24) & 0xFFFFFFFF
+    fine_raw = raw & 0xFFFFFF
+    fine = 0.0
+    for i in range(23, -1, -1):
+        fine += (fine_raw >> i) * 2.0 ** (i - 24)
+        fine_raw &= 1 << 8
+    return float(coarse) + fine
+
+
+def state_vectors(fields) -> list:
+    """build_data_word_dicts + STATE_VECTORS::parse_subcomm_dicts
+    (state_vectors.cpp:19-91): dicts of time, position, velocity, quaternion
+    and angular-rate words, one per complete 64-word cycle, kept while their
+    time stamps increase."""
+    n = len(fields)
+    start = 1
+    while start < n and int(fields[start][SEC["sc_data_word_index"]]) != 1:
+        start += 1
+    out, latest = [], 0.0
+    raw = {k: 0 for k in set(_SUBCOMM)}
+    for i in range(start, n):
+        idx = int(fields[i][SEC["sc_data_word_index"]])
+        if idx == 1 and i != start:
+            d = dict(time=parse_time_stamp(raw["time"]),
+                     position=[int_to_ieee754(raw[k], True) for k in ("px", "py", "pz")],
+                     velocity=[int_to_ieee754(raw[k], False) for k in ("vx", "vy", "vz")],
+                     quaternion=[int_to_ieee754(raw[k], False) for k in ("q0", "q1", "q2", "q3")],
+                     angular_rate=[int_to_ieee754(raw[k], False) for k in ("wx", "wy", "wz")])
+            if d["time"] > latest:
+                latest = d["time"]
+                out.append(d)
+            raw = {k: 0 for k in raw}
+        if 0 <= idx <= 64:
+            k = _SUBCOMM[idx]
+            raw[k] = ((raw[k] << 16) | (int(fields[i][SEC["sc_data_word"]]) & 0xFFFF)) & 0xFFFFFFFFFFFFFFFF
+    return out
+
+
+def effective_velocities(position, v, slant_ranges_):
+    R = np.ascontiguousarray(slant_ranges_, np.float64)
+    out = np.zeros(R.size)
+    lib().orc_effective_velocities(np.ascontiguousarray(position, np.float64), float(v), R, R.size, out)
+    return out
+
+
+def secondary_range_compression(line, v_rel, slant_ranges_, range_freqs, f_dc, az_freq):
+    y = np.array(line, np.complex128, copy=True)
+    n = y.size
+    f = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,)))
+    fac = np.zeros(n)
+    lib().orc_secondary_range_compression(_as_f64(y), n, f(v_rel), f(slant_ranges_), f(range_freqs), f(f_dc),
+                                          float(az_freq), fac)
+    return y, fac
+
+
+def rcmc_phase(line, slant_ranges_, range_freqs, factors):
+    y = np.array(line, np.complex128, copy=True)
+    n = y.size
+    f = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,)))
+    lib().orc_rcmc_phase(_as_f64(y), n, f(slant_ranges_), f(range_freqs), f(factors))
+    return y

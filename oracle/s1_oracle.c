@@ -1026,3 +1026,60 @@ void orc_tables(int* thidx, int* mflag, double* simple, double* nrl, double* sf,
     consts[6] = K_LATCH;
     consts[7] = 16;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Burst geometry: image_formation.cpp:391-476                               */
+/* ------------------------------------------------------------------------ */
+static const double K_SEMI_MAJOR = 6378137.0;    /* structs.h:26 */
+static const double K_SEMI_MINOR = 6356752.3142; /* structs.h:27 */
+
+/* get_effective_velocities (image_formation.cpp:391-419); lat = z / x (Q12). */
+void orc_effective_velocities(const double* pos, double v, const double* R, int n, double* out) {
+    double lat = pos[2] / pos[0];
+    double t1 = K_SEMI_MAJOR * K_SEMI_MAJOR * cos(lat), t2 = K_SEMI_MINOR * K_SEMI_MINOR * sin(lat);
+    double num = t1 * t1 + t2 * t2;
+    double t3 = K_SEMI_MAJOR * cos(lat), t4 = K_SEMI_MINOR * sin(lat);
+    double den = t3 * t3 + t4 * t4;
+    double re = sqrt(num / den);
+    double alt = sqrt(pos[0] * pos[0] + pos[1] * pos[1] + pos[2] * pos[2]);
+    for (int j = 0; j < n; j++) {
+        double r = R[j];
+        double numerator = re * re + alt * alt - r * r;
+        double denominator = 2 * re * alt;
+        double beta = numerator / denominator;
+        double vg = re * v * beta / alt;
+        out[j] = sqrt(v * vg);
+    }
+}
+
+/* apply_secondary_range_compression (image_formation.cpp:422-455) on one
+ * line; returns the factors D in `factors`. */
+void orc_secondary_range_compression(double* line, int n, const double* vrel, const double* R,
+                                     const double* fr, const double* fdc, double az, double* factors) {
+    const double lam = K_WAVELENGTH;
+    for (int j = 0; j < n; j++) {
+        double v = vrel[j];
+        double f = az + fdc[j];
+        double D = sqrt(1 - ((lam * lam) * (f * f)) / (4 * v * v));
+        factors[j] = D;
+        double k = 2.0 * (v * v) * pow(K_F0, 3.0) * (D * D);
+        k /= (K_C * R[j] * (f * f));
+        double ph = -1.0 * K_PI * (fr[j] * fr[j]) / k;
+        double c = cos(ph), s = sin(ph);
+        double xr = line[2 * j], xi = line[2 * j + 1];
+        line[2 * j] = xr * c - xi * s;
+        line[2 * j + 1] = xr * s + xi * c;
+    }
+}
+
+/* apply_range_cell_migration_correction (image_formation.cpp:458-476). */
+void orc_rcmc_phase(double* line, int n, const double* R, const double* fr, const double* factors) {
+    for (int j = 0; j < n; j++) {
+        double shift = (R[j] / factors[j]) - R[j];
+        double ph = -4.0 * K_PI * fr[j] * shift / K_C;
+        double c = cos(ph), s = sin(ph);
+        double xr = line[2 * j], xi = line[2 * j + 1];
+        line[2 * j] = xr * c - xi * s;
+        line[2 * j + 1] = xr * s + xi * c;
+    }
+}

@@ -92,7 +92,15 @@ void orc_tables(int* thidx, int* mflag, double* simple, double* nrl, double* sf,
 double orc_af_entropy_grad(const double* rd, int n_lines, int n_samples, int64_t ld,
                            const double* phase, double* grad_out);
 
+/* image_formation.cpp:391-476 */
+void orc_effective_velocities(const double* pos, double v, const double* R, int n, double* out);
+void orc_secondary_range_compression(double* line, int n, const double* vrel, const double* R,
+                                     const double* fr, const double* fdc, double az, double* factors);
+void orc_rcmc_phase(double* line, int n, const double* R, const double* fr, const double* factors);
+
 #ifdef __cplusplus
 }
 #endif
+
+
 #endif

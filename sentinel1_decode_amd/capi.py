@@ -46,6 +46,11 @@ class BurstDesc(C.Structure):
                 ("out", C.c_void_p), ("ld_out", C.c_int64)]
 
 
+class StateVector(C.Structure):
+    _fields_ = [("time", C.c_double), ("position", C.c_double * 3), ("velocity", C.c_double * 3),
+                ("quaternion", C.c_double * 4), ("angular_rate", C.c_double * 3)]
+
+
 class PacketInfo(C.Structure):
     _fields_ = [("primary", C.c_int64 * 8), ("secondary", C.c_int64 * 46), ("num_quads", C.c_int32),
                 ("baq_mode", C.c_int32), ("range_decimation", C.c_int32), ("rank", C.c_int32),
@@ -97,6 +102,22 @@ _SIGS = [
     ("s1g_sinc_interpolate", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
                                        C.c_void_p]),
     ("s1g_packet_info_from_fields", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("s1g_index_level0", C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    ("s1g_parse_headers", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+    ("s1g_segment_bursts", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
+    ("s1g_state_vectors", C.c_int, [C.c_void_p, C.c_int, C.POINTER(StateVector), C.c_int, C.POINTER(C.c_int)]),
+    ("s1g_interpolate_state", C.c_int, [C.POINTER(StateVector), C.c_int, C.c_double, C.POINTER(StateVector)]),
+    ("s1g_effective_velocities", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                           C.c_void_p, C.c_int64]),
+    ("s1g_secondary_range_compression", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p]),
+    ("s1g_rcmc_phase", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                 C.c_void_p, C.c_void_p]),
+    ("s1g_plan_create", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("s1g_plan_execute", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
+    ("s1g_plan_destroy", C.c_int, [C.c_void_p]),
     ("s1g_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
     ("s1g_force_variant", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("s1g_last_variant", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -155,6 +176,47 @@ def fdbaq_symbol(brc: int, code10: int):
     if rc:
         raise S1GError("bad symbol query")
     return ln.value, m.value
+
+
+def index_level0(stream: np.ndarray) -> np.ndarray:
+    """Start offsets of the packets of a Level-0 byte stream (host)."""
+    stream = np.ascontiguousarray(stream, np.uint8)
+    n = C.c_int64(0)
+    lib().s1g_index_level0(stream.ctypes.data, stream.size, None, 0, C.byref(n))
+    out = np.zeros(n.value, np.uint64)
+    if n.value:
+        lib().s1g_index_level0(stream.ctypes.data, stream.size, out.ctypes.data, out.size, C.byref(n))
+    return out
+
+
+def segment_bursts(fields: np.ndarray, swath_number: int, cal: bool = False):
+    """Per-packet burst index (-1: other swath or dropped) and the burst count."""
+    fields = np.ascontiguousarray(fields, np.int64)
+    out = np.zeros(fields.shape[0], np.int32)
+    nb = C.c_int(0)
+    rc = lib().s1g_segment_bursts(fields.ctypes.data, fields.shape[0], swath_number, int(cal), out.ctypes.data,
+                                  C.byref(nb))
+    if rc:
+        raise S1GError(f"s1g_segment_bursts failed ({rc})")
+    return out, nb.value
+
+
+def state_vectors(fields: np.ndarray):
+    fields = np.ascontiguousarray(fields, np.int64)
+    n = C.c_int(0)
+    lib().s1g_state_vectors(fields.ctypes.data, fields.shape[0], None, 0, C.byref(n))
+    arr = (StateVector * max(n.value, 1))()
+    lib().s1g_state_vectors(fields.ctypes.data, fields.shape[0], arr, n.value, C.byref(n))
+    return list(arr)[: n.value]
+
+
+def interpolate_state(vectors, time: float) -> StateVector:
+    arr = (StateVector * len(vectors))(*vectors)
+    out = StateVector()
+    rc = lib().s1g_interpolate_state(arr, len(vectors), float(time), C.byref(out))
+    if rc:
+        raise S1GError(f"s1g_interpolate_state failed ({rc})")
+    return out
 
 
 # ------------------------------------------------------------- context
@@ -244,6 +306,49 @@ class Context:
         self._check(lib().s1g_sinc_interpolate(self.h, _prec_of(signal), _ptr(signal), signal.shape[0], _ptr(t),
                                                t.shape[0], half_width, _ptr(out)), "sinc_interpolate")
         return out
+
+    # ------------------------------------------------------ Level-0 files
+    def parse_headers(self, d_stream, d_offsets, fields=True):
+        """Device stream + packet offsets -> (fields [n, 54] or None, user-data
+        offsets, lengths, num_quads), all device tensors."""
+        torch = self.torch
+        n = d_offsets.numel()
+        dev = d_stream.device
+        f = torch.empty((n, 54), dtype=torch.int64, device=dev) if fields else None
+        uo = torch.empty(n, dtype=torch.int64, device=dev)
+        ul = torch.empty(n, dtype=torch.int32, device=dev)
+        nq = torch.empty(n, dtype=torch.int32, device=dev)
+        self._bind_stream()
+        self._check(lib().s1g_parse_headers(self.h, _ptr(d_stream), _ptr(d_offsets), n, _ptr(f), _ptr(uo), _ptr(ul),
+                                            _ptr(nq)), "parse_headers")
+        return f, uo, ul, nq
+
+    def effective_velocities(self, positions, velocity_norms, slant_ranges_):
+        torch = self.torch
+        out = torch.empty((positions.shape[0], slant_ranges_.numel()), dtype=torch.float64, device=positions.device)
+        self._bind_stream()
+        self._check(lib().s1g_effective_velocities(self.h, _ptr(positions), _ptr(velocity_norms), positions.shape[0],
+                                                   _ptr(slant_ranges_), slant_ranges_.numel(), _ptr(out),
+                                                   out.stride(0)), "effective_velocities")
+        return out
+
+    def secondary_range_compression(self, x, v_rel, slant_ranges_, range_freqs, f_dc, az_freqs, factors=None):
+        self._bind_stream()
+        self._check(lib().s1g_secondary_range_compression(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x),
+                                                          x.stride(0), _ptr(v_rel), _ptr(slant_ranges_),
+                                                          _ptr(range_freqs), _ptr(f_dc), _ptr(az_freqs),
+                                                          _ptr(factors)), "secondary_range_compression")
+        return x
+
+    def rcmc_phase(self, x, slant_ranges_, range_freqs, factors):
+        self._bind_stream()
+        self._check(lib().s1g_rcmc_phase(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), x.stride(0),
+                                         _ptr(slant_ranges_), _ptr(range_freqs), _ptr(factors)), "rcmc_phase")
+        return x
+
+    def plan(self, prec, n, inverse=False):
+        """An unnormalised FFT plan of length n (the opaque s1g_plan)."""
+        return Plan(self, prec, n, inverse)
 
     # ------------------------------------------------------------- autotuner
     def set_autotune(self, on=True):
@@ -358,6 +463,35 @@ class Context:
                                   _ptr(phase), C.byref(e), _ptr(grad))
         self._check(rc, "af_adjoint")
         return e.value
+
+
+class Plan:
+    """s1g_plan_create / execute / destroy: in-place, unnormalised DFT of
+    rows of length n (the handle that stands in for FFTW's per-row plans)."""
+
+    def __init__(self, ctx, prec, n, inverse=False):
+        self.ctx = ctx
+        self.n = n
+        h = C.c_void_p()
+        ctx._check(lib().s1g_plan_create(ctx.h, prec, n, int(inverse), C.byref(h)), "plan_create")
+        self.h = h
+
+    def execute(self, x):
+        self.ctx._bind_stream()
+        self.ctx._check(lib().s1g_plan_execute(self.h, _ptr(x), x.stride(0) if x.dim() == 2 else self.n,
+                                               x.shape[0] if x.dim() == 2 else 1), "plan_execute")
+        return x
+
+    def close(self):
+        if self.h:
+            lib().s1g_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _prec_of(x):

@@ -5,13 +5,18 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
 #include <stdexcept>
 #include <string>
 
 #include "../../include/s1/image_formation.h"
 #include "../../include/s1/s1_decoder.h"
 #include "../../include/s1/signal_processing.h"
+#include "../../include/s1/state_vectors.h"
 
 namespace {
 
@@ -28,6 +33,33 @@ const char* const kSecondaryFields[46] = {
     "pulse_length", "na_4", "rank", "pri", "swst", "swl", "ssb_flag", "polarisation", "temperature_compensation",
     "na_5", "elevation_beam_address", "na_6", "azimuth_beam_address", "calibration_mode", "na_7",
     "tx_pulse_number", "signal_type", "na_8", "swap", "swath_number", "num_quadratures", "na_9"};
+
+// Swath number -> name (SWATH_NUM_TO_STRING, structs.h:489-559), and the
+// echo swaths among them (ECHO_SWATHS, structs.h:427-432).
+const std::map<int, std::string> kSwathNames = {
+    {0, "S1"}, {1, "S2"}, {2, "S3"}, {3, "S4"}, {4, "S5_N"}, {5, "S5_S"}, {6, "S6"}, {10, "IW1"}, {11, "IW2"},
+    {12, "IW3"}, {20, "EW1"}, {21, "EW2"}, {22, "EW3"}, {23, "EW4"}, {24, "EW5"}, {25, "EW1_100MHz_CAL"},
+    {26, "EW2_100MHz_CAL"}, {27, "EW3_100MHz_CAL"}, {28, "EW4_100MHz_CAL"}, {29, "EW5_100MHz_CAL"}, {30, "WV1"},
+    {31, "WV2"}, {32, "WV1_100MHz_CAL"}, {33, "WV2_100MHz_CAL"}, {36, "S1_100MHz_CAL"}, {37, "S2_100MHz_CAL"},
+    {38, "S3_100MHz_CAL"}, {39, "S4_100MHz_CAL"}, {40, "S5_100MHz_N_CAL"}, {41, "S5_100MHz_S_CAL"},
+    {42, "S6_100MHz_CAL"}, {43, "IW1_100MHz_CAL"}, {44, "IW2_100MHz_CAL"}, {45, "IW3_100MHz_CAL"},
+    {50, "S1_TXCAL"}, {51, "S2_TXCAL"}, {52, "S3_TXCAL"}, {53, "S4_TXCAL"}, {54, "S5_N_TXCAL"},
+    {55, "S5_S_TXCAL"}, {56, "S6_TXCAL"}, {60, "IW1_TXCAL"}, {61, "IW2_TXCAL"}, {62, "IW3_TXCAL"},
+    {70, "EW1_TXCAL"}, {71, "EW2_TXCAL"}, {72, "EW3_TXCAL"}, {73, "EW4_TXCAL"}, {74, "EW5_TXCAL"},
+    {75, "EW1_100MHz_TXCAL"}, {76, "EW2_100MHz_TXCAL"}, {77, "EW3_100MHz_TXCAL"}, {78, "EW4_100MHz_TXCAL"},
+    {79, "EW5_100MHz_TXCAL"}, {80, "WV1_TXCAL"}, {81, "WV2_TXCAL"}, {82, "WV1_100MHz_TXCAL"},
+    {83, "WV2_100MHz_TXCAL"}, {86, "S1_100MHz_TXCAL"}, {87, "S2_100MHz_TXCAL"}, {88, "S3_100MHz_TXCAL"},
+    {89, "S4_100MHz_TXCAL"}, {90, "S5_100MHz_N_TXCAL"}, {91, "S5_100MHz_S_TXCAL"}, {92, "S6_100MHz_TXCAL"},
+    {93, "IW1_100MHz_TXCAL"}, {94, "IW2_100MHz_TXCAL"}, {95, "IW3_100MHz_TXCAL"}};
+const std::set<int> kEchoSwaths = {0, 1, 2, 3, 4, 5, 6, 10, 11, 12, 20, 21, 22, 23, 24, 30, 31};
+const std::set<int> kStripmapSwaths = {0, 1, 2, 3, 4, 5, 6};
+const std::set<int> kIwSwaths = {10, 11, 12};
+
+int swath_number_of(const std::string& name) {
+    for (const auto& kv : kSwathNames)
+        if (kv.second == name) return kv.first;
+    return -1;
+}
 
 const char* const kDecodeErrors[6] = {"ok", "BRC value above 4", "read past the end of the user data",
                                       "Huffman code not found", "magnitude code above the BRC flag",
@@ -191,6 +223,41 @@ D_VEC_1D L0Packet::get_timing_corrections() const {
     if (out.empty()) return out;
     check(s1g_timing_corrections(&_info, out.data()), "timing corrections");
     return out;
+}
+
+double L0Packet::get_time() const {
+    const double coarse = (double)_info.secondary[0], fine = (double)_info.secondary[1];
+    return coarse + (1.0 / fine);
+}
+
+std::string L0Packet::get_swath() const {
+    auto it = kSwathNames.find(_info.swath_number);
+    if (it == kSwathNames.end()) throw std::out_of_range("unknown swath number " + std::to_string(_info.swath_number));
+    return it->second;
+}
+
+char L0Packet::get_rx_polarization() const {
+    switch (_info.secondary[7]) {  // rx_channel_id
+        case 0: return 'V';
+        case 1: return 'H';
+        default: throw std::runtime_error("The rx_channel_id is invalid.");
+    }
+}
+
+char L0Packet::get_tx_polarization() const {
+    const int64_t code = _info.secondary[31];  // polarisation
+    if (code >= 0 && code <= 3) return 'H';
+    if (code >= 4 && code <= 7) return 'V';
+    throw std::runtime_error("The polarization code is invalid.");
+}
+
+D_VEC_1D L0Packet::get_slant_range_times(int num_ranges) const {
+    if (num_ranges <= 0) num_ranges = 2 * _info.num_quads;
+    const double start = _info.swst_us * 1e-6, pri = _info.pri_us * 1e-6, rank = (double)_info.secondary[26];
+    const double fs = _info.fs_mhz * 1e+6, n_rng = 2.0 * _info.num_quads;
+    const double near = rank * pri + start + (320 / (8 * 37.53472224)) * 1e-6;
+    const double far = near + (n_rng / fs);
+    return linspace(near, far, num_ranges);
 }
 
 CF_VEC_1D L0Packet::get_replica_chirp() const {
@@ -443,6 +510,7 @@ CF_VEC_2D run_burst(PACKET_VEC_1D& packets, bool azimuth, bool do_azimuth_fft, c
                     const D_VEC_1D* f_dc, const S1_ECCM_THRESHOLDS* eccm_thr) {
     const DeviceBurst b = pack(packets);
     const int L = b.L, N = b.N;
+    if (eccm_thr) (void)packets[0].get_rx_polarization();  // throws as the reference's ECCM call does
     const int prec = azimuth ? t_burst_prec : S1G_F64;
     const size_t es = prec == S1G_F32 ? 8 : 16;
     const s1g_packet_info& in = packets[0].info();
@@ -533,4 +601,435 @@ CF_VEC_2D azimuth_compress_burst(PACKET_VEC_1D& packets, const D_VEC_1D& effecti
     if (effective_velocities.empty() || doppler_centroids.empty())
         throw std::invalid_argument("effective velocities and Doppler centroids are required");
     return run_burst(packets, true, false, &effective_velocities, &doppler_centroids, eccm);
+}
+
+// --------------------------------------------------------- Level-0 files
+namespace {
+
+std::vector<uint8_t> read_file(const std::string& filename) {
+    std::ifstream f(filename, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + filename);
+    f.seekg(0, std::ios::end);
+    const std::streamoff size = f.tellg();
+    f.seekg(0);
+    std::vector<uint8_t> bytes((size_t)std::max<std::streamoff>(size, 0));
+    if (!bytes.empty()) f.read((char*)bytes.data(), (std::streamsize)bytes.size());
+    return bytes;
+}
+
+// Header fields of packets as rows of S1G_NFIELDS (for the C ABI's
+// segmentation and state-vector functions).
+std::vector<int64_t> field_rows(const PACKET_VEC_1D& packets) {
+    std::vector<int64_t> rows(packets.size() * S1G_NFIELDS);
+    for (size_t k = 0; k < packets.size(); ++k) {
+        const s1g_packet_info& in = packets[k].info();
+        memcpy(&rows[k * S1G_NFIELDS], in.primary, sizeof in.primary);
+        memcpy(&rows[k * S1G_NFIELDS + 8], in.secondary, sizeof in.secondary);
+    }
+    return rows;
+}
+
+}  // namespace
+
+PACKET_VEC_1D get_packets(const std::string& filename, int num_packets) {
+    const std::vector<uint8_t> bytes = read_file(filename);
+    int64_t n = 0;
+    check(s1g_index_level0(bytes.data(), (int64_t)bytes.size(), nullptr, 0, &n), "index");
+    std::vector<uint64_t> offs((size_t)n);
+    check(s1g_index_level0(bytes.data(), (int64_t)bytes.size(), offs.data(), n, &n), "index");
+    PACKET_VEC_1D packets;
+    const int64_t want = num_packets > 0 ? std::min<int64_t>(num_packets, n) : n;
+    packets.reserve((size_t)want);
+    for (int64_t k = 0; k < n && (int64_t)packets.size() < want; ++k) {
+        const uint64_t o = offs[(size_t)k];
+        const size_t len = 7 + (((size_t)bytes[o + 4] << 8) | bytes[o + 5]);
+        try {
+            packets.push_back(L0Packet::from_bytes(UINT8_VEC_1D(bytes.begin() + o, bytes.begin() + o + len),
+                                                   (int)packets.size()));
+        } catch (const std::runtime_error& e) {
+            // the reference reports the packet and reads on (packet.cpp:909-914)
+            std::cout << "Caught a runtime error while decoding packet #" << packets.size() << " - " << e.what()
+                      << std::endl;
+        }
+    }
+    return packets;
+}
+
+PACKET_VEC_1D get_packets_in_swath(const std::string& filename, const std::string& swath) {
+    PACKET_VEC_1D out;
+    for (auto& p : get_packets(filename, 0))
+        if (p.get_swath() == swath) out.push_back(std::move(p));
+    if (out.empty()) throw std::runtime_error("No packets found for swath: " + swath);
+    return out;
+}
+
+PACKET_VEC_2D get_packets_in_bursts(PACKET_VEC_1D& packets, const std::string& swath, bool get_cal_packets) {
+    const int num = swath_number_of(swath);
+    PACKET_VEC_2D bursts;
+    if (num < 0 || packets.empty()) return bursts;
+    const std::vector<int64_t> rows = field_rows(packets);
+    std::vector<int32_t> which(packets.size());
+    int nb = 0;
+    check(s1g_segment_bursts(rows.data(), (int)packets.size(), num, get_cal_packets ? 1 : 0, which.data(), &nb),
+          "burst segmentation");
+    bursts.resize((size_t)nb);
+    for (size_t k = 0; k < packets.size(); ++k)
+        if (which[k] >= 0) bursts[(size_t)which[k]].push_back(packets[k]);
+    return bursts;
+}
+
+PACKET_VEC_2D get_packets_in_bursts(const std::string& filename, const std::string& swath) {
+    PACKET_VEC_1D packets = get_packets_in_swath(filename, swath);
+    return get_packets_in_bursts(packets, swath, false);
+}
+
+// ----------------------------------------------------------- state vectors
+namespace {
+STATE_VECTOR to_state_vector(const s1g_state_vector& v) {
+    STATE_VECTOR s;
+    s.time = v.time;
+    s.position.assign(v.position, v.position + 3);
+    s.velocity.assign(v.velocity, v.velocity + 3);
+    s.quaternions.assign(v.quaternion, v.quaternion + 4);
+    s.angular_rate.assign(v.angular_rate, v.angular_rate + 3);
+    return s;
+}
+s1g_state_vector from_rows(const STATE_VECTORS& s, size_t i) {
+    s1g_state_vector v{};
+    v.time = s.times[i];
+    for (int d = 0; d < 3; ++d) {
+        v.position[d] = s.positions[i][d];
+        v.velocity[d] = s.velocities[i][d];
+        v.angular_rate[d] = s.angular_rates[i][d];
+    }
+    for (int d = 0; d < 4; ++d) v.quaternion[d] = s.quaternions[i][d];
+    return v;
+}
+}  // namespace
+
+STATE_VECTORS::STATE_VECTORS(PACKET_VEC_1D& packets) {
+    const std::vector<int64_t> rows = field_rows(packets);
+    int n = 0;
+    check(s1g_state_vectors(rows.data(), (int)packets.size(), nullptr, 0, &n), "state vectors");
+    std::vector<s1g_state_vector> v((size_t)n);
+    check(s1g_state_vectors(rows.data(), (int)packets.size(), v.data(), n, &n), "state vectors");
+    for (const auto& x : v) {
+        const STATE_VECTOR s = to_state_vector(x);
+        times.push_back(s.time);
+        positions.push_back(s.position);
+        velocities.push_back(s.velocity);
+        quaternions.push_back(s.quaternions);
+        angular_rates.push_back(s.angular_rate);
+    }
+}
+
+STATE_VECTOR STATE_VECTORS::interpolate(const double& time) const {
+    if (times.empty()) throw std::runtime_error("no state vectors to interpolate");
+    std::vector<s1g_state_vector> v(times.size());
+    for (size_t i = 0; i < times.size(); ++i) v[i] = from_rows(*this, i);
+    s1g_state_vector out{};
+    check(s1g_interpolate_state(v.data(), (int)v.size(), time, &out), "interpolate");
+    return to_state_vector(out);
+}
+
+// ------------------------------------------------ image formation (rest)
+std::pair<PACKET_VEC_2D, int> get_azimuth_blocks(PACKET_VEC_1D& packets) {
+    if (packets.empty()) return {PACKET_VEC_2D{}, 0};
+    PACKET_VEC_2D blocks;
+    PACKET_VEC_1D current;
+    int previous = 2 * packets.front().get_num_quads(), max_size = previous;
+    for (const L0Packet& p : packets) {
+        const int size = 2 * p.get_num_quads();
+        if (size != previous && !current.empty()) {
+            blocks.emplace_back(std::move(current));
+            current.clear();
+            previous = size;
+        }
+        max_size = std::max(max_size, size);
+        current.push_back(p);
+    }
+    if (!current.empty()) blocks.emplace_back(std::move(current));
+    return {std::move(blocks), max_size};
+}
+
+D_VEC_1D get_effective_velocities(const D_VEC_1D& position, const double& velocity, const D_VEC_1D& slant_ranges) {
+    if (position.size() != 3) throw std::invalid_argument("position must have 3 components");
+    const int n = (int)slant_ranges.size();
+    D_VEC_1D out((size_t)n);
+    if (n == 0) return out;
+    DevBuf dp(3 * sizeof(double)), dv(sizeof(double)), dr(sizeof(double) * n), dout(sizeof(double) * n);
+    upload(dp.p, position.data(), 3 * sizeof(double));
+    upload(dv.p, &velocity, sizeof(double));
+    upload(dr.p, slant_ranges.data(), sizeof(double) * n);
+    check(s1g_effective_velocities(ctx(), dp.as<double>(), dv.as<double>(), 1, dr.as<double>(), n, dout.as<double>(),
+                                   n),
+          "effective velocities");
+    download(out.data(), dout.p, sizeof(double) * n);
+    return out;
+}
+
+D_VEC_1D apply_secondary_range_compression(CF_VEC_1D& range_line, const D_VEC_1D& effective_velocities,
+                                           const D_VEC_1D& slant_ranges, const D_VEC_1D& range_freqs,
+                                           const D_VEC_1D& doppler_centroids, const double& az_freq) {
+    const size_t n = range_line.size();
+    if (effective_velocities.size() < n || slant_ranges.size() < n || range_freqs.size() < n ||
+        doppler_centroids.size() < n)
+        throw std::out_of_range("geometry vectors shorter than the range line");
+    D_VEC_1D factors(n);
+    if (!n) return factors;
+    DevBuf dx(sizeof(cdouble) * n), dv(8 * n), dr(8 * n), df(8 * n), dc(8 * n), da(8), dfac(8 * n);
+    upload(dx.p, range_line.data(), sizeof(cdouble) * n);
+    upload(dv.p, effective_velocities.data(), 8 * n);
+    upload(dr.p, slant_ranges.data(), 8 * n);
+    upload(df.p, range_freqs.data(), 8 * n);
+    upload(dc.p, doppler_centroids.data(), 8 * n);
+    upload(da.p, &az_freq, 8);
+    check(s1g_secondary_range_compression(ctx(), S1G_F64, 1, (int)n, dx.p, (int64_t)n, dv.as<double>(),
+                                          dr.as<double>(), df.as<double>(), dc.as<double>(), da.as<double>(),
+                                          dfac.as<double>()),
+          "secondary range compression");
+    download(range_line.data(), dx.p, sizeof(cdouble) * n);
+    download(factors.data(), dfac.p, 8 * n);
+    return factors;
+}
+
+void apply_range_cell_migration_correction(CF_VEC_1D& range_line, const D_VEC_1D& slant_ranges,
+                                           const D_VEC_1D& range_freqs, const D_VEC_1D& rcmc_factors) {
+    const size_t n = range_line.size();
+    if (slant_ranges.size() < n || range_freqs.size() < n || rcmc_factors.size() < n)
+        throw std::out_of_range("geometry vectors shorter than the range line");
+    if (!n) return;
+    DevBuf dx(sizeof(cdouble) * n), dr(8 * n), df(8 * n), dfac(8 * n);
+    upload(dx.p, range_line.data(), sizeof(cdouble) * n);
+    upload(dr.p, slant_ranges.data(), 8 * n);
+    upload(df.p, range_freqs.data(), 8 * n);
+    upload(dfac.p, rcmc_factors.data(), 8 * n);
+    check(s1g_rcmc_phase(ctx(), S1G_F64, 1, (int)n, dx.p, (int64_t)n, dr.as<double>(), df.as<double>(),
+                         dfac.as<double>()),
+          "rcmc phase");
+    download(range_line.data(), dx.p, sizeof(cdouble) * n);
+}
+
+// ------------------------------------------------------------- FFT plans
+namespace s1 {
+struct fft_plan_rec {
+    s1g_plan* plan;
+    CF_VEC_1D* row;
+};
+}  // namespace s1
+
+std::vector<fftw_plan> get_fftw_plans(CF_VEC_2D& signals, const int& direction) {
+    if (direction != FFTW_FORWARD && direction != FFTW_BACKWARD)
+        throw std::invalid_argument("direction must be FFTW_FORWARD or FFTW_BACKWARD");
+    std::vector<fftw_plan> plans;
+    plans.reserve(signals.size());
+    try {
+        // each handle binds its row; the context caches the tables per length
+        for (auto& row : signals) {
+            s1g_plan* p = nullptr;
+            if (!row.empty())
+                check(s1g_plan_create(ctx(), S1G_F64, (int)row.size(), direction == FFTW_BACKWARD ? 1 : 0, &p),
+                      "plan");
+            plans.push_back(new s1::fft_plan_rec{p, &row});
+        }
+    } catch (...) {
+        destroy_fftw_plans(plans);
+        throw;
+    }
+    return plans;
+}
+
+void fftw_execute(const fftw_plan plan) {
+    if (!plan || !plan->plan || plan->row->empty()) return;
+    CF_VEC_1D& row = *plan->row;
+    DevBuf d(sizeof(cdouble) * row.size());
+    upload(d.p, row.data(), sizeof(cdouble) * row.size());
+    check(s1g_plan_execute(plan->plan, d.p, (int64_t)row.size(), 1), "plan execute");
+    download(row.data(), d.p, sizeof(cdouble) * row.size());
+}
+
+void destroy_fftw_plans(std::vector<fftw_plan>& plans) {
+    for (auto& p : plans) {
+        if (p) s1g_plan_destroy(p->plan);
+        delete p;
+        p = nullptr;
+    }
+}
+
+// ------------------------------------------------------------ S1_Decoder
+S1_Decoder::S1_Decoder(const std::string& filename) : _filename(filename) {
+    _flat_packets = get_packets(_filename, 0);
+    for (const L0Packet& p : _flat_packets) _swath_counts[p.get_swath()]++;
+    for (const auto& kv : _swath_counts) {
+        const int num = swath_number_of(kv.first);
+        if (kEchoSwaths.count(num))
+            _echo_packets[kv.first] = get_packets_in_bursts(_flat_packets, kv.first, false);
+        else
+            _cal_packets[kv.first] = get_packets_in_bursts(_flat_packets, kv.first, true);
+    }
+    _state_vectors = STATE_VECTORS(_flat_packets);
+}
+
+void S1_Decoder::apply_eccm(const int& detection_threshold, const int& mitigation_threshold) {
+    _use_eccm = true;
+    _eccm_detection_threshold = detection_threshold;
+    _eccm_mitigation_threshold = mitigation_threshold;
+}
+
+// The reference prints the choices and ends the process (std::exit(1),
+// s1_decoder.cpp:47-93); a library throws std::out_of_range with the same
+// text instead.  As there, burst 0 is not checked.
+void S1_Decoder::_validate_request(const std::string& swath, const int& burst) {
+    if (!_swath_counts.count(swath)) {
+        std::string msg = "The requested swath, " + swath + ", does not exist in the data. The available swaths are:";
+        for (const auto& kv : _swath_counts) msg += " " + kv.first;
+        throw std::out_of_range(msg);
+    }
+    if (burst > 0) {
+        const auto& m = _cal_packets.count(swath) ? _cal_packets : _echo_packets;
+        const size_t nb = m.at(swath).size();
+        if ((size_t)burst >= nb)
+            throw std::out_of_range("The requested burst, " + std::to_string(burst) +
+                                    ", does not exist in the data. There are " + std::to_string(nb) +
+                                    " bursts in swath " + swath + ".");
+    }
+}
+
+const PACKET_VEC_2D& S1_Decoder::bursts(const std::string& swath) {
+    _validate_request(swath);
+    return _cal_packets.count(swath) ? _cal_packets.at(swath) : _echo_packets.at(swath);
+}
+
+CF_VEC_2D S1_Decoder::get_burst(const std::string& swath, const int& burst) {
+    _validate_request(swath, burst);
+    PACKET_VEC_1D packets = _echo_packets.at(swath).at((size_t)burst);
+    decode_packets_in_place(packets);
+    CF_VEC_2D out;
+    out.reserve(packets.size());
+    for (auto& p : packets) out.push_back(p.get_signal());
+    return out;
+}
+
+CF_VEC_2D S1_Decoder::get_swath(const std::string& swath) {
+    _validate_request(swath);
+    const int num = swath_number_of(swath);
+    PACKET_VEC_2D groups;
+    if (_cal_packets.count(swath)) {
+        groups = {_cal_packets.at(swath).at(0)};
+    } else if (kIwSwaths.count(num)) {
+        groups = _echo_packets.at(swath);
+    } else if (kStripmapSwaths.count(num)) {
+        groups = get_azimuth_blocks(_echo_packets.at(swath).at(0)).first;
+    } else {
+        throw std::domain_error("Only IW and SM modes are currently supported.");
+    }
+    size_t max_samples = 0;
+    for (const auto& g : groups)
+        if (!g.empty()) max_samples = std::max(max_samples, (size_t)(2 * g[0].get_num_quads()));
+    CF_VEC_2D out;
+    for (auto& g : groups) {
+        PACKET_VEC_1D packets = g;
+        decode_packets_in_place(packets);
+        for (auto& p : packets) {
+            CF_VEC_1D row = p.get_signal();
+            if (!_cal_packets.count(swath)) row.resize(std::max(row.size(), max_samples));
+            out.push_back(std::move(row));
+        }
+    }
+    return out;
+}
+
+CF_VEC_2D S1_Decoder::_range_compress(PACKET_VEC_1D& packets, const bool& do_azimuth_fft) {
+    S1_ECCM_THRESHOLDS thr{(double)_eccm_detection_threshold, (double)_eccm_mitigation_threshold};
+    return range_compress_burst(packets, do_azimuth_fft, _use_eccm ? &thr : nullptr);
+}
+
+D_VEC_1D S1_Decoder::burst_effective_velocities(PACKET_VEC_1D& packets) {
+    const L0Packet& first = packets.at(0);
+    const D_VEC_1D R = first.get_slant_ranges();
+    if (_state_vectors.size() == 0) return D_VEC_1D(R.size(), 7100.0);
+    const double prf = 1.0 / (first.get_pri() * 1e-6);
+    const double L = (double)packets.size();
+    // the reference's row time t0 + i (L / PRF) / PRF at the middle row
+    const double t = first.get_time() + (L / prf) / prf * (double)(packets.size() / 2);
+    const STATE_VECTOR s = _state_vectors.interpolate(t);
+    const double v = std::sqrt(s.velocity[0] * s.velocity[0] + s.velocity[1] * s.velocity[1] +
+                               s.velocity[2] * s.velocity[2]);
+    return get_effective_velocities(s.position, v, R);
+}
+
+CF_VEC_2D S1_Decoder::_azimuth_compress(PACKET_VEC_1D& packets) {
+    S1_ECCM_THRESHOLDS thr{(double)_eccm_detection_threshold, (double)_eccm_mitigation_threshold};
+    const D_VEC_1D v = burst_effective_velocities(packets);
+    return azimuth_compress_burst(packets, v, D_VEC_1D{0.0}, _use_eccm ? &thr : nullptr);
+}
+
+CF_VEC_2D S1_Decoder::get_range_compressed_burst(const std::string& swath, const int& burst,
+                                                 const bool& range_doppler) {
+    _validate_request(swath, burst);
+    PACKET_VEC_1D packets = _echo_packets.at(swath).at((size_t)burst);
+    return _range_compress(packets, range_doppler);
+}
+
+CF_VEC_2D S1_Decoder::get_azimuth_compressed_burst(const std::string& swath, const int& burst) {
+    _validate_request(swath, burst);
+    PACKET_VEC_1D packets = _echo_packets.at(swath).at((size_t)burst);
+    return _azimuth_compress(packets);
+}
+
+namespace {
+// Rows of one group appended to `out`, left-padded with zeros to max_size
+// (the stripmap stacking of s1_decoder.cpp:221-240).
+void append_rows(CF_VEC_2D& out, CF_VEC_2D&& rows, size_t max_size) {
+    for (auto& r : rows) {
+        if (r.size() < max_size) r.insert(r.begin(), max_size - r.size(), cdouble(0.0));
+        out.push_back(std::move(r));
+    }
+}
+}  // namespace
+
+CF_VEC_2D S1_Decoder::get_range_compressed_swath(const std::string& swath, const bool& range_doppler) {
+    _validate_request(swath);
+    const int num = swath_number_of(swath);
+    CF_VEC_2D out;
+    if (kIwSwaths.count(num)) {
+        for (auto& burst : _echo_packets.at(swath)) {
+            PACKET_VEC_1D packets = burst;
+            append_rows(out, _range_compress(packets, range_doppler), 0);
+        }
+    } else if (kStripmapSwaths.count(num)) {
+        auto blocks = get_azimuth_blocks(_echo_packets.at(swath).at(0));
+        for (auto& b : blocks.first) append_rows(out, _range_compress(b, range_doppler), (size_t)blocks.second);
+    } else {
+        throw std::invalid_argument(swath +
+                                    " is not a supported swath type. Only IW and SM imaging modes are supported.");
+    }
+    return out;
+}
+
+CF_VEC_2D S1_Decoder::get_azimuth_compressed_swath(const std::string& swath) {
+    _validate_request(swath);
+    const int num = swath_number_of(swath);
+    CF_VEC_2D out;
+    auto pad20 = [&]() {
+        if (!out.empty()) out.insert(out.end(), 20, CF_VEC_1D(out[0].size()));
+    };
+    if (kIwSwaths.count(num)) {
+        for (auto& burst : _echo_packets.at(swath)) {
+            if (burst.size() < 1000) continue;  // s1_decoder.cpp:322-330
+            PACKET_VEC_1D packets = burst;
+            append_rows(out, _azimuth_compress(packets), 0);
+            pad20();
+        }
+    } else if (kStripmapSwaths.count(num)) {
+        auto blocks = get_azimuth_blocks(_echo_packets.at(swath).at(0));
+        for (auto& b : blocks.first) {
+            append_rows(out, _azimuth_compress(b), (size_t)blocks.second);
+            if (blocks.first.size() > 1) pad20();
+        }
+    } else {
+        throw std::invalid_argument(swath +
+                                    " is not a supported swath type. Only IW and SM imaging modes are supported.");
+    }
+    return out;
 }

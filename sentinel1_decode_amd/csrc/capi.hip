@@ -621,9 +621,9 @@ int s1g_replica_chirp_length(double fs_mhz, double txpl_us) { return (int)floor(
 
 // ---------------------------------------------------------------- headers
 // Field widths of the primary (48 bits) and secondary (496 bits) headers.
-static const int kPrimW[8] = {3, 1, 1, 7, 4, 2, 14, 16};
-static const int kSecW[46] = {32, 16, 32, 32, 8, 1, 3, 4, 32, 8, 16, 32, 32, 1, 2, 5, 8, 8, 8, 8, 1, 15, 1,
-                              15, 24, 3, 5, 24, 24, 24, 1, 3, 2, 2, 4, 2, 10, 2, 1, 5, 4, 3, 1, 8, 16, 8};
+static const int kFieldBits[S1G_NFIELDS] = {S1G_HEADER_FIELD_BITS};
+static const int* const kPrimW = kFieldBits;
+static const int* const kSecW = kFieldBits + 8;
 static const double kFRef = 37.53472224;
 static const double kRangeDecimation[12] = {112.6041667, 100.0925926, 0.0,         83.41049387,
                                             66.72839509, 56.30208336, 50.04629632, 25.02314816,
@@ -811,13 +811,13 @@ int s1g_eccm(s1g_ctx* c, const s1g_eccm_desc* d, int prec, void* data, int64_t l
 // ------------------------------------------------------------ general DSP
 // Rows x n DFT along rows, in place: Stockham passes when n factors over
 // the pass radices, else Bluestein through a power-of-two length.
-static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, int inverse) {
+static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, int inverse, bool normalise = true) {
     if (rows <= 0 || n <= 0) return 0;
     const DevFft* f;
     int rc = get_fft(c, n, &f);
     if (rc) return rc;
     const size_t es = esize(prec);
-    const double scale = inverse ? 1.0 / (double)n : 1.0;
+    const double scale = inverse && normalise ? 1.0 / (double)n : 1.0;
     if (f->plan.npass >= 0) {
         rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 2 * es * (size_t)rows * ld);
         if (rc) return rc;
@@ -886,6 +886,38 @@ extern "C" int s1g_dft(s1g_ctx* c, int prec, int n_rows, int n_cols, void* data,
     if (!data || ld < n_cols) return fail(c, S1G_EARG, "bad DFT buffer%s");
     return axis == 1 ? dft_rows(c, prec, n_rows, n_cols, data, ld, inverse != 0)
                      : dft_cols(c, prec, n_rows, n_cols, data, ld, inverse != 0);
+}
+
+struct s1g_plan {
+    s1g_ctx* ctx;
+    int prec, n, inverse;
+};
+
+extern "C" int s1g_plan_create(s1g_ctx* c, int prec, int n, int inverse, s1g_plan** plan) {
+    if (!c || !plan) return S1G_EARG;
+    *plan = nullptr;
+    if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
+    if (n <= 0) return fail(c, S1G_EARG, "plan length must be positive%s");
+    // build the twiddle tables (and the Bluestein filter) now, as FFTW
+    // plans do their work at creation
+    const DevFft* f;
+    int rc = get_fft(c, n, &f);
+    if (rc) return rc;
+    s1g_plan* p = new (std::nothrow) s1g_plan{c, prec, n, inverse ? 1 : 0};
+    if (!p) return fail(c, S1G_ENOMEM, "plan allocation failed%s");
+    *plan = p;
+    return S1G_OK;
+}
+
+extern "C" int s1g_plan_execute(s1g_plan* p, void* data, int64_t ld, int n_rows) {
+    if (!p) return S1G_EARG;
+    if (n_rows < 0 || !data || ld < p->n) return fail(p->ctx, S1G_EARG, "bad plan execution arguments%s");
+    return dft_rows(p->ctx, p->prec, n_rows, p->n, data, ld, p->inverse, false);
+}
+
+extern "C" int s1g_plan_destroy(s1g_plan* p) {
+    delete p;
+    return S1G_OK;
 }
 
 extern "C" int s1g_transpose(s1g_ctx* c, int prec, int n_rows, int n_cols, const void* src, int64_t ld_src, void* dst,

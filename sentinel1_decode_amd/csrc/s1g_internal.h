@@ -62,6 +62,12 @@ inline float* az_amp_of(AzColumn* cols, int n) {
     return (float*)((char*)cols + ((sizeof(AzColumn) * (size_t)n + sizeof(AzUniform) + 15) & ~(size_t)15));
 }
 
+// Bit widths of the 8 primary (48 bits) and 46 secondary (496 bits) header
+// fields of a space packet, in header order (structs.h:42-115).
+#define S1G_HEADER_FIELD_BITS                                                                                   \
+    3, 1, 1, 7, 4, 2, 14, 16, 32, 16, 32, 32, 8, 1, 3, 4, 32, 8, 16, 32, 32, 1, 2, 5, 8, 8, 8, 8, 1, 15, 1, 15, 24, \
+        3, 5, 24, 24, 24, 1, 3, 2, 2, 4, 2, 10, 2, 1, 5, 4, 3, 1, 8, 16, 8
+
 // Context internals (capi.hip) for entry points defined in other files.
 hipStream_t ctx_stream(const s1g_ctx* c);
 int ctx_device(const s1g_ctx* c);

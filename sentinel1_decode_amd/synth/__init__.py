@@ -360,3 +360,113 @@ def add_rfi(raw: np.ndarray, spec: BurstSpec, seed: int, frac=0.2, run=(10, 50),
             a = math.sqrt(p * 10 ** (inr / 10))
             raw[l] += a * np.exp(1j * (2 * np.pi * f / fs * n + rng.uniform(-np.pi, np.pi)))
     return mask
+
+
+# ------------------------------------------------------------ Level-0 streams
+def _orbit_state(t):
+    """A circular 693 km orbit (inclination 98.18 deg) in an Earth-fixed frame
+    without Earth rotation: ECEF position (m) and velocity (m/s) at time t (s)."""
+    r = 6378137.0 + 693e3
+    mu = 3.986004418e14
+    w = math.sqrt(mu / r ** 3)
+    inc = math.radians(98.18)
+    c, s_ = math.cos(w * t), math.sin(w * t)
+    pos = np.array([r * c, r * s_ * math.cos(inc), r * s_ * math.sin(inc)])
+    vel = np.array([-r * w * s_, r * w * c * math.cos(inc), r * w * c * math.sin(inc)])
+    return pos, vel
+
+
+def _words(value, n_words):
+    return [(value >> (16 * (n_words - 1 - k))) & 0xFFFF for k in range(n_words)]
+
+
+def _subcomm_cycle(t_coarse):
+    """The 64 sub-commutated ancillary words (word indices 1..64) carrying the
+    state vector at time t_coarse (integer seconds): positions as IEEE
+    doubles (4 words each), velocities, quaternion and rates as IEEE floats
+    (2 words), the time stamp as 32 coarse + 24 fine bits (4 words)."""
+    import struct
+    pos, vel = _orbit_state(float(t_coarse))
+    d64 = lambda x: struct.unpack(">Q", struct.pack(">d", float(x)))[0]
+    f32 = lambda x: struct.unpack(">I", struct.pack(">f", float(x)))[0]
+    w = [0] * 65
+    for k, idx in enumerate((1, 5, 9)):
+        w[idx: idx + 4] = _words(d64(pos[k]), 4)
+    for k, idx in enumerate((13, 15, 17)):
+        w[idx: idx + 2] = _words(f32(vel[k]), 2)
+    w[19:23] = _words(int(t_coarse) << 24, 4)                       # pod data stamp
+    for k, (idx, q) in enumerate(zip((23, 25, 27, 29), (0.5, -0.5, 0.5, 0.5))):
+        w[idx: idx + 2] = _words(f32(q), 2)
+    for k, idx in enumerate((31, 33, 35)):
+        w[idx: idx + 2] = _words(f32(1e-3 * (k + 1)), 2)
+    w[37:41] = _words(int(t_coarse) << 24, 4)                       # data time stamp
+    return w
+
+
+@dataclasses.dataclass
+class Level0:
+    """A Level-0 byte stream of several bursts, with what built it: per burst
+    its spec, swath number and the stream indices of its packets, and the
+    orbit's state-vector times."""
+    stream: np.ndarray
+    bursts: list            # [(swath_number, first_packet, n_packets, BurstSpec)]
+    n_packets: int
+
+
+def level0_stream(specs, t0=1406987562, first_pri=243384, lead_packets=0):
+    """Concatenate bursts (BurstSpec, encoded in turn) into one Level-0 stream
+    in acquisition order.  Each packet's headers are rewritten as the
+    instrument writes them: pri_count and space_packet_count count up over
+    the stream (so a swath's consecutive bursts are separated by a PRI gap),
+    azimuth_beam_address ramps within a burst and restarts with the next
+    one, coarse/fine time follow the PRI, and sc_data_word_index cycles
+    1..64 carrying the orbit's state vector (one per second of stream).
+    `lead_packets` > 0 puts that many packets of the first spec's swath
+    before everything (a partial burst the reader must drop)."""
+    chunks, bursts = [], []
+    pkt = 0
+    clock = []  # each packet's time since the stream start (s), one PRI apart
+    items = []
+    if lead_packets:
+        sp0 = dataclasses.replace(specs[0], n_lines=lead_packets, seed=specs[0].seed + 7777)
+        items.append(sp0)
+    items.extend(specs)
+    for sp in items:
+        b = make_burst(sp)
+        n = sp.n_lines
+        bursts.append((sp.swath_number, pkt, n, sp))
+        for l in range(n):
+            raw = bytearray(b.packet(l))
+            p_, s_ = _parse_header_fields(bytes(raw[:68]))
+            clock.append(clock[-1] + 1.0 / sp.prf if clock else 0.0)
+            t = clock[-1]
+            cycle_pos = (pkt + l) % 64
+            sc_idx = cycle_pos + 1
+            # one cycle carries one state vector: the one at its first packet
+            words = _subcomm_cycle(t0 + int(clock[pkt + l - cycle_pos]))
+            s_["pri_count"] = first_pri + pkt + l
+            s_["space_packet_count"] = 240533 + pkt + l
+            s_["azimuth_beam_address"] = 100 + (l * 800) // max(n - 1, 1)
+            s_["coarse_time"] = t0 + int(t)
+            s_["fine_time"] = max(1, int((t - int(t)) * 65536))
+            s_["sc_data_word_index"] = sc_idx
+            s_["sc_data_word"] = words[sc_idx]
+            p_["packet_sequence_count"] = (pkt + l) & 0x3FFF
+            raw[:68] = pack_headers(p_, s_)
+            chunks.append(bytes(raw))
+        pkt += n
+    stream = np.frombuffer(b"".join(chunks), np.uint8).copy()
+    return Level0(stream, bursts, pkt)
+
+
+def _parse_header_fields(h):
+    vals = []
+    bit = 0
+    widths = [3, 1, 1, 7, 4, 2, 14, 16, 32, 16, 32, 32, 8, 1, 3, 4, 32, 8, 16, 32, 32, 1, 2, 5, 8, 8, 8, 8, 1, 15, 1,
+              15, 24, 3, 5, 24, 24, 24, 1, 3, 2, 2, 4, 2, 10, 2, 1, 5, 4, 3, 1, 8, 16, 8]
+    v = int.from_bytes(h, "big")
+    total = 8 * len(h)
+    for w in widths:
+        vals.append((v >> (total - bit - w)) & ((1 << w) - 1))
+        bit += w
+    return dict(zip(PRIMARY_FIELDS, vals[:8])), dict(zip(SECONDARY_FIELDS, vals[8:]))

@@ -15,14 +15,14 @@ def oracle_geometry(burst, v_rel=None, f_dc=None):
     r = oracle.slant_ranges(sec)
     tc = oracle.timing_corrections(sec)
     prf = 1 / (par["pri_us"] * 1e-6)
-    v = np.full(n, burst.spec.v_ms if v_rel is None else v_rel)
-    fd = np.full(n, burst.spec.f_dc if f_dc is None else f_dc)
+    v = np.broadcast_to(np.asarray(burst.spec.v_ms if v_rel is None else v_rel, np.float64), (n,)).copy()
+    fd = np.broadcast_to(np.asarray(burst.spec.f_dc if f_dc is None else f_dc, np.float64), (n,)).copy()
     return dict(par=par, sec=sec, n=n, R=r, tc=tc, prf=prf, v=v, fdc=fd, ref_range=float(r[n // 2]),
                 spacing=float((r[-1] - r[0]) / (n - 1)))
 
 
-def oracle_chain(burst, apply_rcmc=True, eccm=None):
-    g = oracle_geometry(burst)
+def oracle_chain(burst, apply_rcmc=True, eccm=None, v_rel=None, f_dc=None):
+    g = oracle_geometry(burst, v_rel, f_dc)
     raw, st = oracle.decode_packets(burst.stream, burst.ud_offsets, burst.ud_lengths, burst.num_quads)
     out = {"decoded": raw, "status": st}
     x = raw

@@ -15,6 +15,7 @@
 #include "s1/image_formation.h"
 #include "s1/s1_decoder.h"
 #include "s1/signal_processing.h"
+#include "s1/state_vectors.h"
 
 static int failures = 0;
 
@@ -287,10 +288,102 @@ static void packets_and_bursts(const std::string& dir) {
     }
 }
 
+static CF_VEC_1D flat(const CF_VEC_2D& a) {
+    CF_VEC_1D f;
+    for (auto& r : a) f.insert(f.end(), r.begin(), r.end());
+    return f;
+}
+
+// S1_Decoder over a Level-0 file written by the test driver, against the
+// oracle's decode, range-Doppler data and SLC of IW2 burst 1.
+static void level0_file(const std::string& dir) {
+    const auto counts = read_file<int64_t>(dir + "/l0_counts.bin");
+    const auto dec = read_file<cdouble>(dir + "/l0_decoded.bin");
+    const auto rd = read_file<cdouble>(dir + "/l0_rd.bin");
+    const auto slc = read_file<cdouble>(dir + "/l0_slc.bin");
+    const auto vrel = read_file<double>(dir + "/l0_vrel.bin");
+    if (counts.size() != 7 || dec.empty()) {
+        report("level0 fixtures present", false, 0);
+        return;
+    }
+    S1_Decoder d(dir + "/level0.dat");
+    const auto& sc = d.swath_counts();
+    report("S1_Decoder swath counts", sc.at("IW1") == counts[0] && sc.at("IW2") == counts[1] &&
+                                          sc.at("IW3") == counts[2] && sc.size() == 3, 0);
+    report("S1_Decoder bursts per swath", (int64_t)d.bursts("IW1").size() == counts[3] &&
+                                              (int64_t)d.bursts("IW2").size() == counts[4] &&
+                                              (int64_t)d.bursts("IW3").size() == counts[5], 0);
+    report("S1_Decoder state vectors", (int64_t)d.get_state_vectors().size() == counts[6] && counts[6] > 0, 0);
+    bool threw = false;
+    try {
+        (void)d.get_burst("EW1", 0);
+    } catch (const std::out_of_range&) {
+        threw = true;
+    }
+    report("S1_Decoder unknown swath throws", threw, 0);
+    const CF_VEC_1D b = flat(d.get_burst("IW2", 1));
+    size_t mism = b.size() == dec.size() ? 0 : 1;
+    for (size_t i = 0; i < b.size() && i < dec.size(); ++i) mism += std::memcmp(&b[i], &dec[i], sizeof(cdouble)) != 0;
+    report("S1_Decoder::get_burst bit-exact", mism == 0, (double)mism);
+    PACKET_VEC_1D packets = d.bursts("IW2")[1];
+    const D_VEC_1D v = d.burst_effective_velocities(packets);
+    double verr = v.size() == vrel.size() ? 0 : 1;
+    for (size_t j = 0; j < v.size() && j < vrel.size(); ++j) verr = std::max(verr, std::abs(v[j] / vrel[j] - 1));
+    report("S1_Decoder effective velocities", verr < 1e-14, verr);
+    const CF_VEC_1D r = flat(d.get_range_compressed_burst("IW2", 1, true));
+    report("S1_Decoder::get_range_compressed_burst (range-Doppler)", rel(r, rd) < 1e-11, rel(r, rd));
+    const CF_VEC_1D a = flat(d.get_azimuth_compressed_burst("IW2", 1));
+    report("S1_Decoder::get_azimuth_compressed_burst", rel(a, slc) < 1e-10, rel(a, slc));
+    const CF_VEC_2D sw = d.get_azimuth_compressed_swath("IW2");
+    const size_t n0 = d.bursts("IW2")[0].size(), n1 = d.bursts("IW2")[1].size();
+    bool ok = sw.size() == n0 + 20 + n1 + 20;
+    if (ok) {
+        const CF_VEC_2D part(sw.begin() + (long)(n0 + 20), sw.begin() + (long)(n0 + 20 + n1));
+        ok = rel(flat(part), slc) < 1e-10;
+    }
+    report("S1_Decoder::get_azimuth_compressed_swath", ok, 0);
+}
+
+// The reference's image-formation helpers and FFTW-style plans.
+static void geometry_and_plans() {
+    const D_VEC_1D pos = {7071137.0, 0.0, 0.0};
+    const double a = 6378137.0, H = 7071137.0, vv = 7500.0;
+    const D_VEC_1D R = {H - a, 850e3};
+    const D_VEC_1D v = get_effective_velocities(pos, vv, R);
+    report("get_effective_velocities nadir closed form", std::abs(v[0] / (vv * std::sqrt(a / H)) - 1) < 1e-14,
+           std::abs(v[0] / (vv * std::sqrt(a / H)) - 1));
+    CF_VEC_1D line = random_vec(256, 21), keep = line;
+    const D_VEC_1D vr(256, 7100.0), rr(256, 850e3), fr(256, 0.0), dc(256, 0.0);
+    const D_VEC_1D fac = apply_secondary_range_compression(line, vr, rr, fr, dc, 500.0);
+    const double lam = 299792458.0 / 5.405000454334350e9;
+    const double Dw = std::sqrt(1 - (lam * lam * 500.0 * 500.0) / (4 * 7100.0 * 7100.0));
+    report("apply_secondary_range_compression at zero range frequency", rel(line, keep) == 0.0 &&
+                                                                             std::abs(fac[0] - Dw) < 1e-16, 0);
+    apply_range_cell_migration_correction(line, rr, fr, fac);
+    report("apply_range_cell_migration_correction at zero range frequency", rel(line, keep) == 0.0, 0);
+    CF_VEC_2D rows = {random_vec(1480, 31), random_vec(1480, 32)};
+    const CF_VEC_2D orig = rows;
+    std::vector<fftw_plan> fwd = get_fftw_plans(rows);
+    std::vector<fftw_plan> inv = get_fftw_plans(rows, FFTW_BACKWARD);
+    for (auto& p : fwd) fftw_execute(p);
+    const double e1 = rel(rows[1], direct_dft(orig[1], 1480, false));
+    for (auto& p : inv) fftw_execute(p);
+    CF_VEC_1D back = rows[0];
+    for (auto& x : back) x /= 1480.0;  // the plans are unnormalised, like FFTW's
+    destroy_fftw_plans(fwd);
+    destroy_fftw_plans(inv);
+    report("get_fftw_plans forward row", e1 < 1e-11, e1);
+    report("get_fftw_plans backward unnormalised", rel(back, orig[0]) < 1e-12, rel(back, orig[0]));
+}
+
 int main(int argc, char** argv) {
     try {
         building_blocks();
-        if (argc > 1) packets_and_bursts(argv[1]);
+        geometry_and_plans();
+        if (argc > 1) {
+            packets_and_bursts(argv[1]);
+            level0_file(argv[1]);
+        }
     } catch (const std::exception& e) {
         std::printf("FAIL exception %s\n", e.what());
         return 1;

@@ -35,9 +35,47 @@ def test_cpp_dropin(tmp_path):
     np.ascontiguousarray(ch["rc"], np.complex128).tofile(tmp_path / "range_compressed.bin")
     np.ascontiguousarray(ch["slc"], np.complex128).tofile(tmp_path / "slc.bin")
     np.array([spec.v_ms, spec.f_dc], np.float64).tofile(tmp_path / "geometry.bin")
+    _write_level0_case(tmp_path)
     r = subprocess.run([BIN, str(tmp_path)], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     print(r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL PASSED" in r.stdout
-    assert r.stdout.count("PASS ") >= 25
+    assert r.stdout.count("PASS ") >= 40
+
+
+def _write_level0_case(d):
+    """A Level-0 file (IW1/IW2/IW3 bursts, a partial lead burst) for
+    S1_Decoder, and the oracle's decode, range-Doppler data and SLC of IW2
+    burst 1 with the effective velocities the GPU product derives from the
+    file's state vectors."""
+    import types
+
+    import torch
+    import oracle
+    from chain_ref import oracle_chain
+    from sentinel1_decode_amd import capi, pipeline, synth
+    specs = []
+    for k in range(2):
+        for sw, rg, pri in ((10, 6, 22276), (11, 8, 21861), (12, 11, 25868)):
+            specs.append(synth.BurstSpec(n_lines=1050, n_samples=768, rgdec=rg, pri_code=pri, n_targets=6,
+                                         swath_number=sw, seed=300 + 3 * k + sw, clutter_sigma=0.1))
+    s = synth.level0_stream(specs, lead_packets=200)
+    s.stream.tofile(d / "level0.dat")
+    ctx = capi.Context(0)
+    prod = pipeline.Level0Product(s.stream, ctx)
+    idx = prod.packet_indices("IW2", 1)
+    v = prod.burst("IW2", 1).geo.v_rel
+    offs = np.asarray(oracle.index_level0(s.stream), np.uint64)[idx] + 68
+    F = oracle.header_fields(s.stream, [int(o) - 68 for o in offs])
+    spec = [sp for swn, first, n, sp in s.bursts if first == idx[0]][0]
+    ob = types.SimpleNamespace(stream=s.stream, ud_offsets=offs, ud_lengths=(F[:, 7] + 1 - 62).astype(np.uint32),
+                               num_quads=F[:, oracle.SEC["num_quadratures"]].astype(np.int32), spec=spec)
+    ref = oracle_chain(ob, v_rel=v, f_dc=0.0)
+    np.ascontiguousarray(ref["decoded"], np.complex128).tofile(d / "l0_decoded.bin")
+    np.ascontiguousarray(ref["rd"], np.complex128).tofile(d / "l0_rd.bin")
+    np.ascontiguousarray(ref["slc"], np.complex128).tofile(d / "l0_slc.bin")
+    np.ascontiguousarray(v, np.float64).tofile(d / "l0_vrel.bin")
+    counts = [prod.swaths()[n] for n in ("IW1", "IW2", "IW3")] + [prod.n_bursts(n) for n in ("IW1", "IW2", "IW3")]
+    np.asarray(counts + [len(prod.state_vectors)], np.int64).tofile(d / "l0_counts.bin")
+    ctx.close()

@@ -76,7 +76,7 @@ def test_decode_rejects_num_quads_above_maximum(ctx):
     max_quads = 600
     ld = 2 * max_quads + 64  # spare room past the stated maximum in every row
     lens = np.array([len(p) for p in packets], np.uint32)
-    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    d = lambda a: torch.from_numpy(np.array(a)).cuda()
     for prec, dt in ((F64, torch.complex128), (F32, torch.complex64)):
         out = torch.full((3, ld), 7 + 7j, dtype=dt, device="cuda")
         st = torch.full((3,), -9, dtype=torch.int32, device="cuda")
