@@ -21,6 +21,7 @@ template <typename T>
 using VEC_2D = std::vector<std::vector<T>>;
 
 using F_VEC_1D = VEC_1D<float>;
+using F_VEC_2D = VEC_2D<float>;
 using D_VEC_1D = VEC_1D<double>;
 using D_VEC_2D = VEC_2D<double>;
 using INT_VEC_1D = VEC_1D<int>;

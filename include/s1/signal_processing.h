@@ -53,3 +53,16 @@ typedef s1::fft_plan_rec* fftw_plan;
 std::vector<fftw_plan> get_fftw_plans(CF_VEC_2D& signals, const int& direction = FFTW_FORWARD);
 void destroy_fftw_plans(std::vector<fftw_plan>& plans);
 void fftw_execute(const fftw_plan plan);
+
+/* Display scaling (signal_processing.cpp:367-460,1007-1071).  The 2-D
+ * forms run on the GPU (s1g_scale): norm_2d gives (|x| + 1e-12) / max|x|,
+ * or 20 log10 of it, as float; magnitude_2d |x|^2 as float; scale(image,
+ * mode) flattens row-major for mode "norm_log", "norm", "mag", "real" or
+ * "imag" (std::invalid_argument otherwise).  The 1-D forms scale one line
+ * on the host in double, as the reference's do (no 1e-12 offset). */
+std::vector<std::vector<float>> norm_2d(const CF_VEC_2D& complex_values, const bool& log_scale = false);
+F_VEC_2D magnitude_2d(const CF_VEC_2D& complex_values);
+std::vector<float> scale(const CF_VEC_2D& signal, const std::string& scaling_mode);
+D_VEC_1D norm_1d(const CF_VEC_1D& complex_values, const bool& log_scale = false);
+D_VEC_1D magnitude_1d(const CF_VEC_1D& complex_values);
+D_VEC_1D scale(const CF_VEC_1D& signal, const std::string& scaling_mode);

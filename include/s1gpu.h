@@ -410,6 +410,34 @@ int s1g_sinc_interpolate(s1g_ctx* ctx, int prec, const void* signal, int len, co
  * callers holding the reference's header maps, packet.h:142-167). */
 int s1g_packet_info_from_fields(const int64_t* primary, const int64_t* secondary, s1g_packet_info* info);
 
+/* ------------------------------------------------------- image output */
+/* Scaling of a complex image to float for display (scale, norm_2d,
+ * magnitude_2d, signal_processing.cpp:367-460,1007-1071):
+ *   S1G_SCALE_NORM_LOG  20 log10((|x| + 1e-12) / max|x|)
+ *   S1G_SCALE_NORM      (|x| + 1e-12) / max|x|
+ *   S1G_SCALE_MAG       |x|^2
+ *   S1G_SCALE_REAL / S1G_SCALE_IMAG  the real / imaginary part
+ * |x| is evaluated in FP64 for either precision and the result rounded once
+ * to float.  data: device [n_rows][ld] complex of `prec`; out: device float
+ * [n_rows][ld_out]. */
+#define S1G_SCALE_NORM_LOG 0
+#define S1G_SCALE_NORM 1
+#define S1G_SCALE_MAG 2
+#define S1G_SCALE_REAL 3
+#define S1G_SCALE_IMAG 4
+int s1g_scale(s1g_ctx* ctx, int prec, int mode, int n_rows, int n_cols, const void* data, int64_t ld, float* out,
+              int64_t ld_out);
+
+/* complex128 -> complex64, each part rounded to nearest (the cf32 dumps of
+ * image_write_main.cpp:186-324).  Device pointers. */
+int s1g_to_cf32(s1g_ctx* ctx, int n_rows, int n_cols, const void* data, int64_t ld, void* out, int64_t ld_out);
+
+/* Host function: writes a float image as an uncompressed BigTIFF (32-bit
+ * IEEE float samples, one per pixel; _write_tif, image_write.cpp:3-25), with
+ * its rows in reverse order when flip_rows != 0 (the reference writes the
+ * last data row first).  host_data: [n_rows][n_cols] floats. */
+int s1g_write_tiff(const char* path, const float* host_data, int n_rows, int n_cols, int flip_rows);
+
 /* ----------------------------------------------------------- autofocus */
 /* (No reference counterpart.)  rd: range-Doppler data after RCMC and
  * matched filter, [n_lines][ld] (rows = Doppler bins).  phase: device

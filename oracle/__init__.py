@@ -511,3 +511,22 @@ def rcmc_phase(line, slant_ranges_, range_freqs, factors):
     f = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,)))
     lib().orc_rcmc_phase(_as_f64(y), n, f(slant_ranges_), f(range_freqs), f(factors))
     return y
+
+
+# ----------------------------------------------------------- image output
+SCALE_MODES = {"norm_log": 0, "norm": 1, "mag": 2, "real": 3, "imag": 4}
+
+
+def scale_2d(img, mode):
+    """scale / norm_2d / magnitude_2d (signal_processing.cpp:367-460,
+    1007-1071) in FP64, stored as float32: norm adds 1e-12 to |x| before
+    dividing by max |x|; mag is |x|^2."""
+    x = np.asarray(img, np.complex128)
+    if mode in ("real", "imag"):
+        return (x.real if mode == "real" else x.imag).astype(np.float32)
+    a = np.hypot(x.real, x.imag)
+    if mode == "mag":
+        return (a * a).astype(np.float32)
+    mx = a.max()
+    v = a + 1e-12
+    return (20.0 * np.log10(v / mx) if mode == "norm_log" else v / mx).astype(np.float32)

@@ -118,6 +118,10 @@ _SIGS = [
     ("s1g_plan_create", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     ("s1g_plan_execute", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
     ("s1g_plan_destroy", C.c_int, [C.c_void_p]),
+    ("s1g_scale", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                            C.c_int64]),
+    ("s1g_to_cf32", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
+    ("s1g_write_tiff", C.c_int, [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
     ("s1g_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
     ("s1g_force_variant", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("s1g_last_variant", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -217,6 +221,18 @@ def interpolate_state(vectors, time: float) -> StateVector:
     if rc:
         raise S1GError(f"s1g_interpolate_state failed ({rc})")
     return out
+
+
+SCALE_MODES = {"norm_log": 0, "norm": 1, "mag": 2, "real": 3, "imag": 4}
+
+
+def write_tiff(path, img: np.ndarray, flip_rows=True):
+    """A float image as an uncompressed BigTIFF (last row first by default,
+    as the reference writes it)."""
+    img = np.ascontiguousarray(img, np.float32)
+    rc = lib().s1g_write_tiff(str(path).encode(), img.ctypes.data, img.shape[0], img.shape[1], int(flip_rows))
+    if rc:
+        raise S1GError(f"s1g_write_tiff failed ({rc})")
 
 
 # ------------------------------------------------------------- context
@@ -345,6 +361,24 @@ class Context:
         self._check(lib().s1g_rcmc_phase(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), x.stride(0),
                                          _ptr(slant_ranges_), _ptr(range_freqs), _ptr(factors)), "rcmc_phase")
         return x
+
+    def scale(self, x, mode):
+        """Display scaling of a complex image to float (mode: norm_log, norm,
+        mag, real, imag)."""
+        torch = self.torch
+        out = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+        self._bind_stream()
+        self._check(lib().s1g_scale(self.h, _prec_of(x), SCALE_MODES[mode], x.shape[0], x.shape[1], _ptr(x),
+                                    x.stride(0), _ptr(out), out.stride(0)), "scale")
+        return out
+
+    def to_cf32(self, x):
+        torch = self.torch
+        out = torch.empty(x.shape, dtype=torch.complex64, device=x.device)
+        self._bind_stream()
+        self._check(lib().s1g_to_cf32(self.h, x.shape[0], x.shape[1], _ptr(x), x.stride(0), _ptr(out),
+                                      out.stride(0)), "to_cf32")
+        return out
 
     def plan(self, prec, n, inverse=False):
         """An unnormalised FFT plan of length n (the opaque s1g_plan)."""

@@ -14,6 +14,7 @@
 #include <string>
 
 #include "../../include/s1/image_formation.h"
+#include "../../include/s1/image_write.h"
 #include "../../include/s1/s1_decoder.h"
 #include "../../include/s1/signal_processing.h"
 #include "../../include/s1/state_vectors.h"
@@ -1032,4 +1033,111 @@ CF_VEC_2D S1_Decoder::get_azimuth_compressed_swath(const std::string& swath) {
                                     " is not a supported swath type. Only IW and SM imaging modes are supported.");
     }
     return out;
+}
+
+
+// ------------------------------------------------------------ image output
+namespace {
+int scale_mode(const std::string& m) {
+    if (m == "norm_log") return S1G_SCALE_NORM_LOG;
+    if (m == "norm") return S1G_SCALE_NORM;
+    if (m == "mag") return S1G_SCALE_MAG;
+    if (m == "real") return S1G_SCALE_REAL;
+    if (m == "imag") return S1G_SCALE_IMAG;
+    throw std::invalid_argument(m + " is not a valid scaling mode.");
+}
+
+// The image on the GPU, scaled, back as row-major floats.
+std::vector<float> scale_dev(const CF_VEC_2D& img, int mode, int* rows_out, int* cols_out) {
+    const int rows = (int)img.size();
+    const int cols = rows ? (int)img[0].size() : 0;
+    for (const auto& r : img)
+        if ((int)r.size() != cols) throw std::invalid_argument("scaling needs rows of equal length");
+    *rows_out = rows;
+    *cols_out = cols;
+    std::vector<float> out((size_t)rows * cols);
+    if (out.empty()) return out;
+    DevBuf d(sizeof(cdouble) * out.size()), o(sizeof(float) * out.size());
+    upload_rows(img, cols, d.as<cdouble>());
+    check(s1g_scale(ctx(), S1G_F64, mode, rows, cols, d.p, cols, o.as<float>(), cols), "scale");
+    download(out.data(), o.p, sizeof(float) * out.size());
+    return out;
+}
+
+std::vector<std::vector<float>> unflatten(const std::vector<float>& v, int rows, int cols) {
+    std::vector<std::vector<float>> out((size_t)rows);
+    for (int r = 0; r < rows; ++r) out[(size_t)r].assign(v.begin() + (long)r * cols, v.begin() + (long)(r + 1) * cols);
+    return out;
+}
+}  // namespace
+
+std::vector<std::vector<float>> norm_2d(const CF_VEC_2D& complex_values, const bool& log_scale) {
+    int r, c;
+    const std::vector<float> v = scale_dev(complex_values, log_scale ? S1G_SCALE_NORM_LOG : S1G_SCALE_NORM, &r, &c);
+    return unflatten(v, r, c);
+}
+
+F_VEC_2D magnitude_2d(const CF_VEC_2D& complex_values) {
+    int r, c;
+    const std::vector<float> v = scale_dev(complex_values, S1G_SCALE_MAG, &r, &c);
+    return unflatten(v, r, c);
+}
+
+std::vector<float> scale(const CF_VEC_2D& signal, const std::string& scaling_mode) {
+    int r, c;
+    return scale_dev(signal, scale_mode(scaling_mode), &r, &c);
+}
+
+D_VEC_1D norm_1d(const CF_VEC_1D& complex_values, const bool& log_scale) {
+    D_VEC_1D n(complex_values.size());
+    double mx = 0;
+    for (size_t i = 0; i < n.size(); ++i) {
+        n[i] = std::abs(complex_values[i]);
+        if (n[i] > mx) mx = n[i];
+    }
+    for (auto& v : n) v = log_scale ? 20 * std::log10(v / mx) : v / mx;
+    return n;
+}
+
+D_VEC_1D magnitude_1d(const CF_VEC_1D& complex_values) {
+    D_VEC_1D m(complex_values.size());
+    for (size_t i = 0; i < m.size(); ++i) m[i] = std::pow(std::abs(complex_values[i]), 2.0);
+    return m;
+}
+
+D_VEC_1D scale(const CF_VEC_1D& signal, const std::string& scaling_mode) {
+    const int mode = scale_mode(scaling_mode);
+    if (mode == S1G_SCALE_NORM_LOG) return norm_1d(signal, true);
+    if (mode == S1G_SCALE_NORM) return norm_1d(signal, false);
+    if (mode == S1G_SCALE_MAG) return magnitude_1d(signal);
+    D_VEC_1D out(signal.size());
+    for (size_t i = 0; i < out.size(); ++i) out[i] = mode == S1G_SCALE_REAL ? signal[i].real() : signal[i].imag();
+    return out;
+}
+
+void _write_tif(std::vector<float>& img_data, const int& rows, const int& cols, const std::string out_filename) {
+    if ((size_t)rows * (size_t)cols > img_data.size()) throw std::invalid_argument("image smaller than rows x cols");
+    if (s1g_write_tiff(out_filename.c_str(), img_data.data(), rows, cols, 1) != S1G_OK)
+        throw std::runtime_error("cannot write " + out_filename);
+}
+
+void write_tif(CF_VEC_2D& img_data, const std::string& out_filename, const std::string& scaling_mode) {
+    const int rows = (int)img_data.size(), cols = rows ? (int)img_data[0].size() : 0;
+    std::vector<float> scaled = scale(img_data, scaling_mode);
+    _write_tif(scaled, rows, cols, out_filename);
+}
+
+void write_cf32(const CF_VEC_2D& data, const std::string& out_filename) {
+    std::ofstream f(out_filename, std::ios::binary);
+    if (!f) throw std::runtime_error("Unable to open file " + out_filename + " for writing.");
+    const int rows = (int)data.size(), cols = rows ? (int)data[0].size() : 0;
+    if (rows && cols) {
+        DevBuf d(sizeof(cdouble) * (size_t)rows * cols), o(8 * (size_t)rows * cols);
+        upload_rows(data, cols, d.as<cdouble>());
+        check(s1g_to_cf32(ctx(), rows, cols, d.p, cols, o.p, cols), "to_cf32");
+        std::vector<std::complex<float>> h((size_t)rows * cols);
+        download(h.data(), o.p, 8 * h.size());
+        f.write((const char*)h.data(), (std::streamsize)(8 * h.size()));
+    }
+    if (!f) throw std::runtime_error("Failed to write to file " + out_filename + ".");
 }

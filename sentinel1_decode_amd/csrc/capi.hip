@@ -470,6 +470,11 @@ int ctx_error(s1g_ctx* c, int code, const char* msg) {
     c->err = msg;
     return code;
 }
+int ctx_scratch(s1g_ctx* c, size_t bytes, void** p) {
+    int rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, bytes);
+    *p = rc ? nullptr : c->dft_ws;
+    return rc;
+}
 int ctx_rcmc_variant(s1g_ctx* c, int prec, int n_lines, int n_samples, int apply_rcmc, int fallback) {
     const int nvar = rcmc_variants(prec);
     int v = fallback;

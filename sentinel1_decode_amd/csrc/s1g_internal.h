@@ -73,6 +73,9 @@ hipStream_t ctx_stream(const s1g_ctx* c);
 int ctx_device(const s1g_ctx* c);
 // Sets the context's error message and returns code.
 int ctx_error(s1g_ctx* c, int code, const char* msg);
+// The context's general scratch buffer, grown to at least `bytes` (shared
+// with the DFT entry points; valid until the next call that uses it).
+int ctx_scratch(s1g_ctx* c, size_t bytes, void** p);
 // RCMC launch variant for a shape without timing any: the forced variant,
 // else the tuner's cached choice, else `fallback`; recorded as the stage's
 // last variant.

@@ -13,6 +13,7 @@
 #include <string>
 
 #include "s1/image_formation.h"
+#include "s1/image_write.h"
 #include "s1/s1_decoder.h"
 #include "s1/signal_processing.h"
 #include "s1/state_vectors.h"
@@ -374,6 +375,33 @@ static void geometry_and_plans() {
     destroy_fftw_plans(inv);
     report("get_fftw_plans forward row", e1 < 1e-11, e1);
     report("get_fftw_plans backward unnormalised", rel(back, orig[0]) < 1e-12, rel(back, orig[0]));
+    // display scaling and writers
+    CF_VEC_2D img = {random_vec(40, 41), random_vec(40, 42), random_vec(40, 43)};
+    double mx = 0;
+    for (auto& r : img)
+        for (auto& v : r) mx = std::max(mx, std::abs(v));
+    const std::vector<float> n = scale(img, "norm");
+    double worst = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 40; ++j)
+            worst = std::max(worst, std::abs(n[i * 40 + j] - (float)((std::abs(img[i][j]) + 1e-12) / mx)) /
+                                        (double)n[i * 40 + j]);
+    report("scale norm", worst < 1.2e-7, worst);
+    const F_VEC_2D m = magnitude_2d(img);
+    report("magnitude_2d", std::abs(m[1][5] - (float)std::norm(img[1][5])) <= 1e-6f * m[1][5], 0);
+    bool threw = false;
+    try {
+        (void)scale(img, "loudness");
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    report("scale unknown mode throws", threw, 0);
+    write_cf32(img, "/tmp/s1_dropin_test.cf32");
+    const auto cf = read_file<std::complex<float>>("/tmp/s1_dropin_test.cf32");
+    report("write_cf32", cf.size() == 120 && cf[41] == std::complex<float>(img[1][1]), 0);
+    write_tif(img, "/tmp/s1_dropin_test.tif", "norm_log");
+    const auto tif = read_file<uint8_t>("/tmp/s1_dropin_test.tif");
+    report("write_tif BigTIFF", tif.size() > 16 + 480 && tif[0] == 'I' && tif[2] == 43, 0);
 }
 
 int main(int argc, char** argv) {

@@ -141,3 +141,23 @@ def test_fft_plan_unnormalised(ctx, n):
         want = np.fft.ifft(x, axis=1) * n if inverse else np.fft.fft(x, axis=1)
         assert rel_l2(d.cpu().numpy(), want) < 1e-13
         p.close()
+
+
+@pytest.mark.parametrize("mode", ["norm_log", "norm", "mag", "real", "imag"])
+@pytest.mark.parametrize("dt", [torch.complex128, torch.complex64])
+def test_scale_vs_oracle(ctx, mode, dt):
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal((300, 1000)) + 1j * rng.standard_normal((300, 1000))) * 1e4
+    x[17, 400] = 1e7 + 1e7j
+    d = torch.from_numpy(x).to(dt).cuda()
+    got = ctx.scale(d, mode).cpu().numpy()
+    ref = oracle.scale_2d(d.cpu().numpy(), mode)
+    # FP64 |x| (hypot) on both sides, rounded once to float: within an ulp
+    assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref)))
+
+
+def test_to_cf32(ctx):
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((50, 333)) + 1j * rng.standard_normal((50, 333))
+    got = ctx.to_cf32(torch.from_numpy(x).cuda()).cpu().numpy()
+    assert np.array_equal(got, x.astype(np.complex64))
