@@ -301,6 +301,95 @@ int s1g_rcmc_matched_filter(s1g_ctx* ctx, const s1g_azimuth_desc* desc, int prec
 int s1g_rcmc_matched_filter_noalloc(s1g_ctx* ctx, const s1g_azimuth_desc* desc, int prec, const void* in,
                                     int64_t ld_in, void* out, int64_t ld_out);
 
+/* ------------------------------------------------- TOPS (IW) focusing */
+/* The TOPS branch of S1_Decoder::_azimuth_compress (s1_decoder.cpp:511-701):
+ * Doppler centroid estimation, azimuth-frequency unfolding, the per-line
+ * filter, azimuth-time unfolding.  Device arrays are [lines][samples]
+ * (range samples contiguous); all stages compute in FP64 (prec selects the
+ * element type of the complex data: S1G_F32 or S1G_F64). */
+
+/* L0Packet::get_azimuth_beam_angle (packet.cpp:83-90):
+ * linspace(-0.018, 0.018, 1024)[address], radians (host). */
+double s1g_azimuth_beam_angle(int address);
+
+/* get_doppler_centroid_rate (doppler.cpp:224-232): -2 v k / lambda with the
+ * steering rate k = (angle(last) - angle(first)) / (pri n_packets), Hz/s
+ * (host). */
+double s1g_doppler_centroid_rate(double pri_us, int first_beam_address, int last_beam_address, int n_packets,
+                                 double velocity);
+
+/* get_doppler_centroid (doppler.cpp:146-221) of a range-compressed burst
+ * (device, [n_lines][n_samples]): the burst deramped at `rate` over 1.06 s,
+ * lag-one phase per range sample, block means over n_blocks range blocks
+ * (30 in the reference), get_unwrapped_estimates, a quadratic fit over the
+ * slant-range times linspace(t_near, t_far, .) (L0Packet::
+ * get_slant_range_times of the first packet) and its value at every
+ * sample: dc[n_samples] (device, FP64).  The lag sums are serial per sample
+ * and the rest runs on the host in FP64, so the result is that of the
+ * reference's arithmetic on the same input. */
+int s1g_doppler_centroid(s1g_ctx* ctx, int prec, int n_lines, int n_samples, const void* data, int64_t ld,
+                         double rate, double prf, double t_near, double t_far, int n_blocks, double* dc);
+
+/* Lines of azimuth_frequency_ufr's output: floor(|rate T / prf| n_lines). */
+int s1g_tops_frequency_lines(int n_lines, double rate, double burst_duration, double prf);
+
+/* azimuth_frequency_ufr (image_formation.cpp:122-235): per range sample,
+ * the fftshifted azimuth spectrum tiled |rate T / prf| times, deramped
+ * exp(-i pi f^2 / rate), low-passed to floor(doppler_bandwidth) bins moved to
+ * the start, re-ramped.  in: [n_lines][n_samples], out:
+ * [s1g_tops_frequency_lines(...)][n_samples]; device. */
+int s1g_azimuth_frequency_ufr(s1g_ctx* ctx, int prec, int n_lines, int n_samples, const void* in, int64_t ld_in,
+                              double rate, double burst_duration, double prf, double doppler_bandwidth, void* out,
+                              int64_t ld_out);
+
+/* The TOPS branch of the per-line loop of S1_Decoder::_azimuth_compress
+ * (s1_decoder.cpp:626-677) on n_lines lines in place: range FFT, secondary
+ * range compression at az_freqs[i] + f_dc[j] with v_rel from
+ * get_effective_velocities(positions[i], velocity_norms[i], R), unnormalised
+ * range IFFT, division by sqrt((ref_range / R_j)^3), RCMC phase
+ * exp(-4 i pi range_freqs[j] (R_j / D - R_j) / c), azimuth filter
+ * exp(4 i pi R_j D / lambda) exp(2 i pi (az_freqs[i] + f_dc[j]) t_corr[j]);
+ * fm[i][j] = -2 v^2 D^3 / (lambda R_j) (the azimuth FM rate).  Device:
+ * positions [n_lines][3], velocity_norms, az_freqs [n_lines], slant_ranges,
+ * range_freqs, f_dc, t_corr [n_samples], fm [n_lines][n_samples], FP64. */
+int s1g_tops_filter_lines(s1g_ctx* ctx, int prec, int n_lines, int n_samples, void* data, int64_t ld,
+                          const double* positions, const double* velocity_norms, const double* az_freqs,
+                          const double* slant_ranges, const double* range_freqs, const double* f_dc,
+                          const double* t_corr, double ref_range, double* fm);
+
+/* The scalar set-up of azimuth_time_ufr (image_formation.cpp:238-292). */
+typedef struct {
+    double bandwidth;        /* nr prf - doppler_bandwidth, Hz              */
+    double focused_time;     /* s                                           */
+    double n_ref;            /* f_dc / k_a at the reference point, s        */
+    double ka_mean;          /* |mean FM rate| of range sample 0, Hz/s      */
+    int32_t num_tiles;       /* time replicas                               */
+    int32_t shape;           /* num_tiles * n_lines                         */
+    int32_t lpf_start, lpf_end;
+    int32_t downsample_shape;
+    int32_t output_shape;    /* floor(prf T)                                */
+    int32_t crop_start, crop_end; /* output lines [crop_start, crop_end)    */
+} s1g_time_ufr_params;
+
+/* azimuth_time_ufr's set-up from f_dc[n_samples] and fm[n_lines][n_samples]
+ * (device).  n_ref divides f_dc[n_samples / 2] by fm at line n_samples / 2,
+ * sample n_lines / 2 (the reference's index on its transposed array) where
+ * those are in range, else by fm at line n_lines / 2, sample n_samples / 2
+ * (the reference reads past its array there).  S1G_EARG when the low-pass
+ * window or the crop leaves the arrays. */
+int s1g_azimuth_time_ufr_params(s1g_ctx* ctx, int n_lines, int n_samples, const double* f_dc, const double* fm,
+                                double rate, double burst_duration, double prf, double doppler_bandwidth,
+                                int swath_number, s1g_time_ufr_params* out);
+
+/* azimuth_time_ufr (image_formation.cpp:238-388) on an azimuth-compressed
+ * burst in: [n_lines][n_samples]: per range sample, tiled num_tiles times,
+ * deramped with the FM rate linearly resampled to the tiled length,
+ * low-passed, quadratically resampled to downsample_shape lines, re-ramped
+ * and cropped.  out: [crop_end - crop_start][n_samples]; device. */
+int s1g_azimuth_time_ufr(s1g_ctx* ctx, int prec, int n_lines, int n_samples, const void* in, int64_t ld_in,
+                         const double* f_dc, const double* fm, double rate, double burst_duration, double prf,
+                         double doppler_bandwidth, int swath_number, void* out, int64_t ld_out);
+
 /* ---------------------------------------------------------------- ECCM */
 /* RFI detection and notch, in place (eccm, signal_processing.cpp:913-1004,
  * as called from s1_decoder.cpp:470-474 with fft_size 128, stride 64).

@@ -101,6 +101,21 @@ def _bind(L):
     L.orc_rcmc_phase.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
     L.orc_set_threads.argtypes = [C.c_int]
     L.orc_set_threads.restype = C.c_int
+    L.orc_dc_wrapped.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, _dp]
+    L.orc_dc_unwrap.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, C.c_double, _dp]
+    L.orc_dc_unwrap.restype = C.c_int
+    L.orc_tops_freq_shape.argtypes = [C.c_int, C.c_double]
+    L.orc_tops_freq_shape.restype = C.c_int
+    L.orc_az_freq_ufr.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  _dp]
+    L.orc_tops_rows.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp]
+    L.orc_linear_resample_many.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp]
+    L.orc_quadratic_resample.argtypes = [_dp, C.c_int, C.c_int, _dp]
+    L.orc_tops_time_params.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, _dp, np.ctypeslib.ndpointer(np.int32)]
+    L.orc_tops_time_params.restype = C.c_int
+    L.orc_az_time_ufr.argtypes = [_dp, C.c_int, C.c_int, C.c_int64, _dp, _dp, C.c_double, _dp,
+                                  np.ctypeslib.ndpointer(np.int32), _dp]
     return L
 
 
@@ -530,3 +545,201 @@ def scale_2d(img, mode):
     mx = a.max()
     v = a + 1e-12
     return (20.0 * np.log10(v / mx) if mode == "norm_log" else v / mx).astype(np.float32)
+
+
+# -------------------------------------------------------------- TOPS (IW)
+WAVELENGTH = 299792458.0 / 5.405000454334350e9
+
+
+def azimuth_beam_angle(address):
+    """L0Packet::get_azimuth_beam_angle (packet.cpp:83-90):
+    linspace(-0.018, 0.018, 1024)[address] in radians."""
+    return float(linspace(-0.018, 0.018, 1024)[int(address)])
+
+
+def doppler_centroid_rate(pri_us, first_address, last_address, n_packets, velocity):
+    """get_doppler_centroid_rate (doppler.cpp:224-232): -2 v k_psi / lambda,
+    k_psi = (last - first beam angle) / (pri n_packets)."""
+    pri = pri_us * 1e-6
+    steer = (azimuth_beam_angle(last_address) - azimuth_beam_angle(first_address)) / (pri * n_packets)
+    return (-2 * velocity * steer) / WAVELENGTH
+
+
+def dc_wrapped(rc, rate, prf):
+    """get_wrapped_estimates over dce_preconditioning (doppler.cpp:4-143)
+    with the burst duration fixed at 1.06 s as get_doppler_centroid passes it
+    (:161): one estimate per range column."""
+    x = _c128(rc)
+    L, n = x.shape
+    out = np.zeros(n)
+    lib().orc_dc_wrapped(_as_f64(x), L, n, n, float(rate), float(prf), out)
+    return out
+
+
+def dc_unwrap(wrapped, t_near, t_far, prf, n_blocks=30):
+    """The rest of get_doppler_centroid (doppler.cpp:146-221): block means,
+    get_unwrapped_estimates (:42-95), polyfit (signal_processing.cpp:159-214)
+    and its value at every range sample; the range times are
+    L0Packet::get_slant_range_times (packet.cpp:304-318) = linspace(t_near,
+    t_far, count)."""
+    w = np.ascontiguousarray(wrapped, np.float64)
+    n = w.size
+    bt = linspace(t_near, t_far, n_blocks)
+    t = linspace(t_near, t_far, n)
+    out = np.zeros(n)
+    if lib().orc_dc_unwrap(w, n, n_blocks, bt, t, float(prf), out) != 0:
+        raise ValueError("need at least two range blocks")
+    return out
+
+
+def doppler_centroid(rc, rate, prf, t_near, t_far, n_blocks=30):
+    """get_doppler_centroid (doppler.cpp:146-221)."""
+    return dc_unwrap(dc_wrapped(rc, rate, prf), t_near, t_far, prf, n_blocks)
+
+
+def tops_replicas(rate, burst_duration, prf):
+    """num_replicas = |rate T / prf| (image_formation.cpp:142)."""
+    return abs(rate * burst_duration / prf)
+
+
+def az_freq_ufr(rc, rate, burst_duration, prf, bw, nr=None, inv_rate=None):
+    """azimuth_frequency_ufr (image_formation.cpp:122-235): [floor(nr L)][n].
+    nr / inv_rate override |rate T / prf| and 1 / rate (drives the chain
+    without its ramp, for closed-form checks)."""
+    x = _c128(rc)
+    L, n = x.shape
+    nr = tops_replicas(rate, burst_duration, prf) if nr is None else float(nr)
+    ir = 1.0 / rate if inv_rate is None else float(inv_rate)
+    shape = lib().orc_tops_freq_shape(L, nr)
+    mid = shape // 2
+    start, end = int(mid - np.floor(bw) / 2), int(mid + np.floor(bw) / 2)
+    if shape < 1 or start < 0 or end > shape:
+        raise ValueError("the low-pass window leaves the spectrum")
+    out = np.zeros((shape, n), np.complex128)
+    lib().orc_az_freq_ufr(_as_f64(x), L, n, n, nr, ir, float(prf), float(bw), _as_f64(out))
+    return out
+
+
+def tops_rows(data, v_rel, slant_ranges_, range_freqs, f_dc, az_freqs, t_corr, ref_range):
+    """The TOPS branch of the Doppler-row loop of S1_Decoder::_azimuth_compress
+    (s1_decoder.cpp:626-677) on every line: range FFT, secondary range
+    compression, unnormalised range IFFT, amplitude weight, RCMC phase,
+    azimuth filter.  Returns (lines, azimuth FM rate [lines][n])."""
+    y = np.array(data, np.complex128, copy=True)
+    rows, n = y.shape
+    f = lambda a: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,)))
+    v = np.ascontiguousarray(np.broadcast_to(np.asarray(v_rel, np.float64), (rows, n)))
+    fm = np.zeros((rows, n))
+    lib().orc_tops_rows(_as_f64(y), rows, n, n, v, f(slant_ranges_), f(range_freqs), f(f_dc),
+                        np.ascontiguousarray(az_freqs, np.float64), f(t_corr), float(ref_range), fm)
+    return y, fm
+
+
+def linear_resample(x, n_out):
+    """linear_resample (signal_processing.cpp:32-63), real rows."""
+    a = np.ascontiguousarray(np.atleast_2d(x), np.float64)
+    out = np.zeros((a.shape[0], n_out))
+    lib().orc_linear_resample_many(a, a.shape[1], int(n_out), a.shape[0], out)
+    return out.reshape(-1) if np.ndim(x) == 1 else out
+
+
+def quadratic_resample(x, n_out):
+    """quadratic_resample (signal_processing.cpp:109-155), one complex row."""
+    a = _c128(x)
+    if a.size < 3 and a.size != n_out:
+        raise ValueError("Need at least 3 points for quadratic interpolation.")
+    out = np.zeros(int(n_out), np.complex128)
+    lib().orc_quadratic_resample(_as_f64(a), a.size, int(n_out), _as_f64(out))
+    return out
+
+
+def fm_reference(fm):
+    """The FM rate n_ref divides by (image_formation.cpp:252): the reference
+    reads az_fm_rate[n_az / 2][n / 2] of the transposed [n][n_az] array, i.e.
+    line n / 2, range sample n_az / 2; where that is out of range (as for
+    IW bursts, n > 2 n_az) it reads past the vector, and the value taken
+    here is the evident intent, line n_az / 2 at range sample n / 2."""
+    n_az, n = fm.shape
+    if n_az // 2 < n and n // 2 < n_az:
+        return float(fm[n // 2, n_az // 2])
+    return float(fm[n_az // 2, n // 2])
+
+
+def tops_time_params(fm, dc, nr, prf, burst_duration, bw, swath):
+    """The scalar set-up of azimuth_time_ufr (image_formation.cpp:238-292)."""
+    fm = np.ascontiguousarray(fm, np.float64)
+    n_az, n = fm.shape
+    d = np.zeros(4)
+    k = np.zeros(8, np.int32)
+    rc = lib().orc_tops_time_params(np.ascontiguousarray(fm[:, 0]), n_az, n, float(dc[n // 2]), fm_reference(fm),
+                                    float(nr), float(prf), float(burst_duration), float(bw), int(swath), d, k)
+    keys_d = ("bandwidth", "focused_time", "n_ref", "ka_mean")
+    keys_k = ("num_tiles", "shape", "lpf_start", "lpf_end", "downsample_shape", "output_shape", "crop_start",
+              "crop_end")
+    p = dict(zip(keys_d, d.tolist()))
+    p.update(zip(keys_k, k.tolist()))
+    p["ok"] = rc == 0
+    return p, d, k
+
+
+def az_time_ufr(ac, dc, fm, rate, burst_duration, prf, bw, swath, nr=None):
+    """azimuth_time_ufr (image_formation.cpp:238-388): [crop rows][n].  nr
+    overrides |rate T / prf| (for closed-form checks)."""
+    x = _c128(ac)
+    n_az, n = x.shape
+    nr = tops_replicas(rate, burst_duration, prf) if nr is None else float(nr)
+    p, d, k = tops_time_params(fm, dc, nr, prf, burst_duration, bw, swath)
+    if not p["ok"]:
+        raise ValueError(f"azimuth time UFR set-up out of range: {p}")
+    out = np.zeros((int(k[7] - k[6]), n), np.complex128)
+    lib().orc_az_time_ufr(_as_f64(x), n_az, n, n, np.ascontiguousarray(dc, np.float64),
+                          np.ascontiguousarray(fm, np.float64), float(rate), d, k, _as_f64(out))
+    return out, p
+
+
+def _lerp(a, b, t):
+    """std::lerp as libstdc++ writes it (the interpolation of
+    interpolate_vector, state_vectors.cpp:142-153)."""
+    if (a <= 0 and b >= 0) or (a >= 0 and b <= 0):
+        return t * b + (1 - t) * a
+    if t == 1:
+        return b
+    x = a + t * (b - a)
+    if (t > 1) == (b > a):
+        return x if b < x else b
+    return x if b > x else b
+
+
+def interpolate_state(times, positions, velocities, time):
+    """STATE_VECTORS::interpolate (state_vectors.cpp:127-200): the first /
+    last vector outside the span, else a bisection for the bracketing pair
+    (a vector at exactly the time gives a == b and t = 0 / 0) and lerp of
+    every component.  Returns (position, velocity).  A pivot equal to the
+    time ends the bisection with the window's start twice (a == b, so t is
+    (time - a) / 0)."""
+    times = [float(t) for t in times]
+    if time > times[-1]:
+        return list(positions[-1]), list(velocities[-1])
+    if time < times[0]:
+        return list(positions[0]), list(velocities[0])
+    lo, hi = 0, len(times)
+    while True:
+        size = hi - lo
+        if size == 1:
+            a_i, b_i = lo, hi
+            break
+        pivot = size // 2
+        tp = times[lo + pivot]
+        if tp == time:
+            a_i = b_i = lo
+            break
+        if tp > time:
+            hi = lo + pivot
+        else:
+            lo = lo + pivot
+    a, b = times[a_i], times[b_i]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        t = float(np.float64(time - a) / np.float64(b - a))
+    pos = [_lerp(float(p), float(q), t) for p, q in zip(positions[a_i], positions[b_i])]
+    vel = [_lerp(float(p), float(q), t) for p, q in zip(velocities[a_i], velocities[b_i])]
+    return pos, vel

@@ -1083,3 +1083,431 @@ void orc_rcmc_phase(double* line, int n, const double* R, const double* fr, cons
         line[2 * j + 1] = xr * s + xi * c;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* TOPS: doppler.cpp:4-232, image_formation.cpp:87-388, s1_decoder.cpp:511-701 */
+/* ------------------------------------------------------------------------ */
+static void cmul(double ar, double ai, double br, double bi, double* r, double* i) {
+    *r = ar * br - ai * bi;
+    *i = ar * bi + ai * br;
+}
+
+/* get_wrapped_estimates over dce_preconditioning (doppler.cpp:4-138): the
+ * burst deramped with exp(i PI rate (t - 0.53)^2), t = linspace(0, 1.06, L),
+ * then per range column -prf arg(sum_i x_i conj(x_{i+1})) / (2 PI). */
+void orc_dc_wrapped(const double* rc, int L, int n, int64_t ld, double rate, double prf, double* wrapped) {
+    double* t = (double*)malloc(sizeof(double) * (size_t)L);
+    double* dr = (double*)malloc(sizeof(double) * 2 * (size_t)L);
+    orc_linspace(0.0, 1.06, L, t);
+    for (int i = 0; i < L; i++) {
+        double d = t[i] - (0.5 * 1.06);
+        double ph = K_PI * rate * (d * d);
+        dr[2 * i] = cos(ph);
+        dr[2 * i + 1] = sin(ph);
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; j++) {
+        double sr = 0.0, si = 0.0;
+        for (int i = 0; i < L - 1; i++) {
+            double ar, ai, br, bi, pr, pi_;
+            cmul(rc[2 * (ld * i + j)], rc[2 * (ld * i + j) + 1], dr[2 * i], dr[2 * i + 1], &ar, &ai);
+            cmul(rc[2 * (ld * (i + 1) + j)], rc[2 * (ld * (i + 1) + j) + 1], dr[2 * (i + 1)], dr[2 * (i + 1) + 1],
+                 &br, &bi);
+            cmul(ar, ai, br, -bi, &pr, &pi_);
+            sr += pr;
+            si += pi_;
+        }
+        wrapped[j] = -prf * atan2(si, sr) / (2 * K_PI);
+    }
+    free(t);
+    free(dr);
+}
+
+/* The rest of get_doppler_centroid (doppler.cpp:141-218): block means of
+ * the wrapped estimates, get_unwrapped_estimates (:43-90, its
+ * floor(|exp(i theta)|) included), a quadratic polyfit over the blocks'
+ * range times and its value at every range sample. */
+static void orc_polyfit2(const double* x, const double* y, int N, double* c);
+int orc_dc_unwrap(const double* wrapped, int n, int n_blocks, const double* blk_times, const double* times,
+                  double prf, double* dc) {
+    if (n_blocks < 2 || n < n_blocks) return -1;
+    int bs = n / n_blocks;
+    double* fine = (double*)malloc(sizeof(double) * (size_t)n_blocks);
+    double* bt = (double*)malloc(sizeof(double) * (size_t)n_blocks);
+    for (int b = 0; b < n_blocks; b++) {
+        int s = b * bs, e = s + bs < n ? s + bs : n - 1;
+        double sum = 0.0;
+        for (int k = s; k < e; k++) sum += wrapped[k];
+        fine[b] = sum / ((double)e - (double)s);
+        bt[b] = blk_times[b] - blk_times[0];
+    }
+    int m = n_blocks * 8;
+    float dt = (float)(bt[1] - bt[0]);
+    double* F = (double*)calloc(2 * (size_t)m, sizeof(double));
+    for (int b = 0; b < n_blocks; b++) {
+        double th = 2.0 * K_PI * fine[b] / prf;
+        F[2 * b] = floor(hypot(cos(th), sin(th)));
+    }
+    orc_fft(F, m, 0);
+    int best = 0;
+    double bv = -1.0;
+    for (int k = 0; k < m; k++) {
+        double a = hypot(F[2 * k], F[2 * k + 1]);
+        double v = a * a;
+        if (v > bv) {
+            bv = v;
+            best = k;
+        }
+    }
+    /* fftfreq(m, dt) */
+    double val = 1.0 / (m * (double)dt);
+    int half = (m - 1) / 2 + 1;
+    double v = best < half ? best * val : (best - m) * val;
+    double a = v / (double)dt;
+    double bb = atan2(F[2 * best + 1], F[2 * best]) / (2.0 * K_PI);
+    double* un = (double*)malloc(sizeof(double) * (size_t)n_blocks);
+    for (int b = 0; b < n_blocks; b++) {
+        double th1 = 2.0 * K_PI * fine[b] / prf, th2 = -(a * bt[b] + bb);
+        double r = atan2(sin(th1) * cos(th2) + cos(th1) * sin(th2), cos(th1) * cos(th2) - sin(th1) * sin(th2));
+        un[b] = (a * bt[b] + bb + r / (2.0 * K_PI)) * prf;
+    }
+    double c[3];
+    orc_polyfit2(bt, un, n_blocks, c);
+    double t0 = times[0];
+    for (int j = 0; j < n; j++) {
+        double t = times[j] - t0;
+        dc[j] = (c[0] + c[1] * t + c[2] * t * t);
+    }
+    free(fine);
+    free(bt);
+    free(F);
+    free(un);
+    return 0;
+}
+
+/* polyfit (signal_processing.cpp:159-213): normal equations of degree 2,
+ * Gaussian elimination with partial pivoting. */
+static void orc_polyfit2(const double* x, const double* y, int N, double* c) {
+    double Sx[5] = {0}, Sy[3] = {0};
+    for (int i = 0; i < N; ++i) {
+        double xi = 1.0;
+        for (int j = 0; j <= 4; ++j) {
+            if (j <= 2) Sy[j] += xi * y[i];
+            Sx[j] += xi;
+            xi *= x[i];
+        }
+    }
+    double A[3][3] = {{Sx[0], Sx[1], Sx[2]}, {Sx[1], Sx[2], Sx[3]}, {Sx[2], Sx[3], Sx[4]}};
+    double b[3] = {Sy[0], Sy[1], Sy[2]};
+    for (int i = 0; i < 3; ++i) {
+        for (int k = i + 1; k < 3; ++k) {
+            if (fabs(A[k][i]) > fabs(A[i][i])) {
+                for (int j = 0; j < 3; ++j) {
+                    double tmp = A[i][j];
+                    A[i][j] = A[k][j];
+                    A[k][j] = tmp;
+                }
+                double tmp = b[i];
+                b[i] = b[k];
+                b[k] = tmp;
+            }
+        }
+        for (int k = i + 1; k < 3; ++k) {
+            double f = A[k][i] / A[i][i];
+            for (int j = i; j < 3; ++j) A[k][j] -= f * A[i][j];
+            b[k] -= f * b[i];
+        }
+    }
+    for (int i = 2; i >= 0; --i) {
+        c[i] = b[i];
+        for (int j = i + 1; j < 3; ++j) c[i] -= A[i][j] * c[j];
+        c[i] /= A[i][i];
+    }
+}
+
+/* azimuth_frequency_ufr (image_formation.cpp:122-235): per range column,
+ * the fftshifted azimuth spectrum tiled nr = |rate T / prf| times
+ * (get_tiled_signal, :87-119), deramped by exp(-i PI f^2 / rate) over
+ * f = linspace(-nr prf / 2, nr prf / 2, shape), low-passed to the
+ * floor(bw) bins around the centre (moved to the start), re-ramped.
+ * out: [shape][n], shape = floor(nr L).  nr and 1 / rate are separate
+ * arguments so that the chain can be driven without the ramp. */
+int orc_tops_freq_shape(int L, double nr) { return (int)floor(nr * L); }
+void orc_az_freq_ufr(const double* in, int L, int n, int64_t ld, double nr, double inv_rate, double prf, double bw,
+                     double* out) {
+    double B = nr * prf / 2.0;
+    int shape = (int)floor(nr * L);
+    int mid = shape / 2;
+    int start = (int)(mid - (floor(bw) / 2)), end = (int)(mid + (floor(bw) / 2));
+    double* f = (double*)malloc(sizeof(double) * (size_t)shape);
+    orc_linspace(-B, B, shape, f);
+    int whole = (int)floor(nr);
+    int rest = floor(nr) != nr ? (int)floor((nr - floor(nr)) * L) : 0;
+    int filled = whole * L + rest < shape ? whole * L + rest : shape;
+    double sc = 1.0 / (double)shape;
+#pragma omp parallel
+    {
+        double* col = (double*)malloc(sizeof(double) * 2 * (size_t)L);
+        double* sh = (double*)malloc(sizeof(double) * 2 * (size_t)L);
+        double* u = (double*)malloc(sizeof(double) * 2 * (size_t)shape);
+#pragma omp for schedule(dynamic, 8)
+        for (int j = 0; j < n; j++) {
+            for (int i = 0; i < L; i++) {
+                col[2 * i] = in[2 * (ld * i + j)];
+                col[2 * i + 1] = in[2 * (ld * i + j) + 1];
+            }
+            orc_fft(col, L, 0);
+            /* fftshift = std::rotate(begin, end - L / 2, end) */
+            for (int i = 0; i < L; i++) {
+                int src = (i + L - L / 2) % L;
+                sh[2 * i] = col[2 * src];
+                sh[2 * i + 1] = col[2 * src + 1];
+            }
+            for (int a = 0; a < shape; a++) {
+                double xr = 0.0, xi = 0.0;
+                if (a < filled) {
+                    xr = sh[2 * (a % L)];
+                    xi = sh[2 * (a % L) + 1];
+                }
+                double ph = (-K_PI * inv_rate) * (f[a] * f[a]);
+                double c = cos(ph), s = sin(ph);
+                u[2 * a] = c * xr - s * xi;
+                u[2 * a + 1] = c * xi + s * xr;
+            }
+            orc_fft(u, shape, 0);
+            for (int a = 0; a < shape; a++) {
+                if (a < end - start) {
+                    u[2 * a] = u[2 * (start + a)];
+                    u[2 * a + 1] = u[2 * (start + a) + 1];
+                } else {
+                    u[2 * a] = 0.0;
+                    u[2 * a + 1] = 0.0;
+                }
+            }
+            orc_fft(u, shape, 1);
+            for (int a = 0; a < shape; a++) {
+                double ur = u[2 * a] * sc, ui = u[2 * a + 1] * sc;
+                double ph = (K_PI * inv_rate) * (f[a] * f[a]);
+                double c = cos(ph), s = sin(ph);
+                out[2 * ((int64_t)a * n + j)] = c * ur - s * ui;
+                out[2 * ((int64_t)a * n + j) + 1] = c * ui + s * ur;
+            }
+        }
+        free(col);
+        free(sh);
+        free(u);
+    }
+    free(f);
+}
+
+/* The TOPS branch of the Doppler-row loop (s1_decoder.cpp:616-677): per
+ * row, unnormalised range FFT, secondary range compression, unnormalised
+ * range IFFT, amplitude weight, RCMC phase ramp, azimuth filter; the
+ * azimuth FM rate -2 v^2 D^3 / (lambda R) per sample into fm. */
+void orc_tops_rows(double* data, int rows, int n, int64_t ld, const double* vrel, const double* R,
+                   const double* fr, const double* fdc, const double* faz, const double* tc, double ref_range,
+                   double* fm) {
+    const double lam = K_WAVELENGTH;
+#pragma omp parallel
+    {
+        double* fac = (double*)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(dynamic, 4)
+        for (int i = 0; i < rows; i++) {
+            double* x = data + 2 * ld * (int64_t)i;
+            orc_fft(x, n, 0);
+            orc_secondary_range_compression(x, n, vrel + (int64_t)i * n, R, fr, fdc, faz[i], fac);
+            orc_fft(x, n, 1);
+            for (int j = 0; j < n; j++) {
+                double amp = sqrt(pow(ref_range / R[j], 3.0));
+                x[2 * j] /= amp;
+                x[2 * j + 1] /= amp;
+            }
+            orc_rcmc_phase(x, n, R, fr, fac);
+            for (int j = 0; j < n; j++) {
+                double v = vrel[(int64_t)i * n + j], D = fac[j];
+                double ph2 = ((2.0 * K_PI) * (faz[i] + fdc[j])) * tc[j];
+                double ph1 = (((4.0 * K_PI) * R[j]) * D) / lam;
+                double c1 = cos(ph1), s1 = sin(ph1), c2 = cos(ph2), s2 = sin(ph2);
+                double hr = c1 * c2 - s1 * s2, hi = c1 * s2 + s1 * c2;
+                double xr = x[2 * j], xi = x[2 * j + 1];
+                x[2 * j] = xr * hr - xi * hi;
+                x[2 * j + 1] = xr * hi + xi * hr;
+                fm[(int64_t)i * n + j] = -2 * pow(v, 2.0) * pow(D, 3.0) / (lam * R[j]);
+            }
+        }
+        free(fac);
+    }
+}
+
+/* linear_resample (signal_processing.cpp:32-63) of a real sequence, as
+ * written: a running abscissa x += dx_out walked over the input segments,
+ * the last output the last input. */
+static void orc_linear_resample(const double* in, int n_in, int n_out, double* out) {
+    if (n_in == n_out) {
+        memcpy(out, in, sizeof(double) * (size_t)n_in);
+        return;
+    }
+    double x = 0.0, dx_in = 1.0 / (n_in - 1), dx_out = 1.0 / (n_out - 1);
+    int index = 0;
+    for (int i = 0; i < n_in - 1; i++) {
+        double x0 = i * dx_in, x1 = (i + 1) * dx_in, y0 = in[i], y1 = in[i + 1];
+        while (x < x1 && index < n_out) {
+            out[index] = y0 + ((x - x0) * (y1 - y0) / (x1 - x0));
+            x += dx_out;
+            index += 1;
+        }
+    }
+    out[n_out - 1] = in[n_in - 1];
+}
+void orc_linear_resample_many(const double* in, int n_in, int n_out, int count, double* out) {
+    for (int k = 0; k < count; k++) orc_linear_resample(in + (int64_t)k * n_in, n_in, n_out, out + (int64_t)k * n_out);
+}
+
+/* quadratic_resample (signal_processing.cpp:109-155): three-point Lagrange
+ * through i0 = max(i - 1, 0), i, min(i + 1, n_in - 1) with i = min(int(x /
+ * dx_in), n_in - 2); outputs whose points coincide (x < dx_in) are 0 / 0. */
+void orc_quadratic_resample(const double* in, int n_in, int n_out, double* out) {
+    if (n_in == n_out) {
+        memcpy(out, in, sizeof(double) * 2 * (size_t)n_in);
+        return;
+    }
+    double dx_in = 1.0 / (n_in - 1), dx_out = 1.0 / (n_out - 1);
+    for (int j = 0; j < n_out; ++j) {
+        double x = j * dx_out;
+        int i = (int)(x / dx_in);
+        if (i > n_in - 2) i = n_in - 2;
+        int i0 = i - 1 > 0 ? i - 1 : 0, i1 = i, i2 = i + 1 < n_in - 1 ? i + 1 : n_in - 1;
+        double x0 = i0 * dx_in, x1 = i1 * dx_in, x2 = i2 * dx_in;
+        double L0 = (x - x1) * (x - x2) / ((x0 - x1) * (x0 - x2));
+        double L1 = (x - x0) * (x - x2) / ((x1 - x0) * (x1 - x2));
+        double L2 = (x - x0) * (x - x1) / ((x2 - x0) * (x2 - x1));
+        out[2 * j] = L0 * in[2 * i0] + L1 * in[2 * i1] + L2 * in[2 * i2];
+        out[2 * j + 1] = L0 * in[2 * i0 + 1] + L1 * in[2 * i1 + 1] + L2 * in[2 * i2 + 1];
+    }
+}
+
+/* The scalar set-up of azimuth_time_ufr (image_formation.cpp:238-292) for a
+ * burst of n_az (frequency-UFR) lines and n range samples.  fm_col0 is the
+ * azimuth FM rate of range sample 0 on every line, fm_ref the rate at the
+ * reference point of n_ref = f_dc[n / 2] / fm_ref.
+ *   d: [bandwidth, focused_time, n_ref, ka_mean]
+ *   k: [num_tiles, shape, lpf_start, lpf_end, downsample_shape,
+ *       output_shape, crop_start, crop_end]
+ * Returns 0, or -1 when the low-pass window or the crop leaves the arrays. */
+int orc_tops_time_params(const double* fm_col0, int n_az, int n, double dc_mid, double fm_ref, double nr, double prf,
+                         double T, double bw, int swath, double* d, int* k) {
+    double bandwidth = (nr * prf) - bw;
+    double sum = 0.0;
+    for (int a = 0; a < n_az; a++) sum += fm_col0[a];
+    double ka_mean = fabs(sum) / n_az;
+    double focused_time = T + ((bandwidth - (2 * bw)) / ka_mean);
+    double n_ref = dc_mid / fm_ref;
+    if (swath != 11)
+        focused_time *= 0.9;
+    else
+        focused_time *= 1.05;
+    int pos = (int)ceil((n_ref + focused_time / 2) / T);
+    int neg = (int)floor((n_ref - focused_time / 2) / T);
+    int tiles = pos - neg;
+    int ext = swath == 11 ? (int)floor(bw * 1.15) : (int)floor(bw * 0.26);
+    int out_shape = (int)floor(prf * T);
+    int ds = out_shape + ext;
+    int shape = tiles * n_az;
+    int mid = shape / 2;
+    int start = (int)(mid - bandwidth / 2), end = (int)(mid + bandwidth / 2);
+    int cmid = ds / 2;
+    d[0] = bandwidth;
+    d[1] = focused_time;
+    d[2] = n_ref;
+    d[3] = ka_mean;
+    k[0] = tiles;
+    k[1] = shape;
+    k[2] = start;
+    k[3] = end;
+    k[4] = ds;
+    k[5] = out_shape;
+    k[6] = cmid - (out_shape / 2);
+    k[7] = cmid + (out_shape / 2);
+    if (tiles < 1 || start < 0 || end > shape || end < start || ds < 3 || shape < 3 || k[6] < 0 || k[7] > ds)
+        return -1;
+    return 0;
+}
+
+/* azimuth_time_ufr (image_formation.cpp:238-388) after the set-up: per
+ * range column r, the n_az lines tiled k[0] times, deramped with the FM rate
+ * resampled to the tiled length, low-passed, resampled (quadratic) to the
+ * downsample length, re-ramped and cropped to rows [k[6], k[7]).
+ * in: [n_az][n] (ld), dc[n], fm: [n_az][n] (row-major, FM rate of line a at
+ * range r), out: [k[7] - k[6]][n]. */
+void orc_az_time_ufr(const double* in, int n_az, int n, int64_t ld, const double* dc, const double* fm, double rate,
+                     const double* d, const int* k, double* out) {
+    int tiles = k[0], shape = k[1], start = k[2], end = k[3], ds = k[4], c0 = k[6], c1 = k[7];
+    double ft = d[1], n_ref = d[2];
+    double* t1 = (double*)malloc(sizeof(double) * (size_t)shape);
+    double* t2 = (double*)malloc(sizeof(double) * (size_t)ds);
+    orc_linspace(-ft / 2, ft / 2, shape, t1);
+    orc_linspace(n_ref - ft / 2, n_ref + ft / 2, ds, t2);
+    double sc = 1.0 / (double)shape;
+#pragma omp parallel
+    {
+        double* col = (double*)malloc(sizeof(double) * (size_t)n_az);
+        double* ka = (double*)malloc(sizeof(double) * (size_t)(shape > ds ? shape : ds));
+        double* u = (double*)malloc(sizeof(double) * 2 * (size_t)shape);
+        double* v = (double*)malloc(sizeof(double) * 2 * (size_t)ds);
+#pragma omp for schedule(dynamic, 4)
+        for (int r = 0; r < n; r++) {
+            for (int a = 0; a < n_az; a++) col[a] = fm[(int64_t)a * n + r];
+            orc_linear_resample(col, n_az, shape, ka);
+            double fdc = dc[r];
+            (void)tiles;
+            for (int a = 0; a < shape; a++) {
+                int src = a % n_az;
+                double xr = in[2 * (ld * src + r)], xi = in[2 * (ld * src + r) + 1];
+                double kt = -ka[a] * rate / (rate - ka[a]);
+                double dct = -fdc / ka[a];
+                double t = dct + t1[a];
+                double p1 = ((-1.0 * K_PI) * kt) * pow(t, 2.0);
+                double p2 = (((2.0 * K_PI) * (kt + ka[a])) * dct) * t;
+                double ph = p1 + p2;
+                double c = cos(ph), s = sin(ph);
+                u[2 * a] = c * xr - s * xi;
+                u[2 * a + 1] = c * xi + s * xr;
+            }
+            orc_fft(u, shape, 0);
+            for (int a = 0; a < shape; a++) {
+                if (a < end - start) {
+                    u[2 * a] = u[2 * (start + a)];
+                    u[2 * a + 1] = u[2 * (start + a) + 1];
+                } else {
+                    u[2 * a] = 0.0;
+                    u[2 * a + 1] = 0.0;
+                }
+            }
+            orc_fft(u, shape, 1);
+            for (int a = 0; a < shape; a++) {
+                u[2 * a] *= sc;
+                u[2 * a + 1] *= sc;
+            }
+            orc_quadratic_resample(u, shape, ds, v);
+            orc_linear_resample(col, n_az, ds, ka);
+            for (int j = c0; j < c1; j++) {
+                double kt = -ka[j] * rate / (rate - ka[j]);
+                double dct = -fdc / ka[j];
+                double t = dct + t2[j];
+                double p1 = ((1.0 * K_PI) * kt) * pow(t, 2.0);
+                double p2 = (((2.0 * K_PI) * (kt + ka[j])) * dct) * t;
+                double ph = p1 - p2;
+                double c = cos(ph), s = sin(ph);
+                double xr = v[2 * j], xi = v[2 * j + 1];
+                out[2 * ((int64_t)(j - c0) * n + r)] = c * xr - s * xi;
+                out[2 * ((int64_t)(j - c0) * n + r) + 1] = c * xi + s * xr;
+            }
+        }
+        free(col);
+        free(ka);
+        free(u);
+        free(v);
+    }
+    free(t1);
+    free(t2);
+}

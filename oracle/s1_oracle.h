@@ -98,6 +98,24 @@ void orc_secondary_range_compression(double* line, int n, const double* vrel, co
                                      const double* fr, const double* fdc, double az, double* factors);
 void orc_rcmc_phase(double* line, int n, const double* R, const double* fr, const double* factors);
 
+/* TOPS: doppler.cpp:4-232, image_formation.cpp:87-388, s1_decoder.cpp:511-701,
+ * signal_processing.cpp:32-214 */
+void orc_dc_wrapped(const double* rc, int L, int n, int64_t ld, double rate, double prf, double* wrapped);
+int orc_dc_unwrap(const double* wrapped, int n, int n_blocks, const double* blk_times, const double* times,
+                  double prf, double* dc);
+int orc_tops_freq_shape(int L, double nr);
+void orc_az_freq_ufr(const double* in, int L, int n, int64_t ld, double nr, double inv_rate, double prf, double bw,
+                     double* out);
+void orc_tops_rows(double* data, int rows, int n, int64_t ld, const double* vrel, const double* R,
+                   const double* fr, const double* fdc, const double* faz, const double* tc, double ref_range,
+                   double* fm);
+void orc_linear_resample_many(const double* in, int n_in, int n_out, int count, double* out);
+void orc_quadratic_resample(const double* in, int n_in, int n_out, double* out);
+int orc_tops_time_params(const double* fm_col0, int n_az, int n, double dc_mid, double fm_ref, double nr, double prf,
+                         double T, double bw, int swath, double* d, int* k);
+void orc_az_time_ufr(const double* in, int n_az, int n, int64_t ld, const double* dc, const double* fm, double rate,
+                     const double* d, const int* k, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,16 @@ class PacketInfo(C.Structure):
                 ("txprr_mhz_us", C.c_double), ("txpsf_mhz", C.c_double), ("fs_mhz", C.c_double)]
 
 
+class TimeUfrParams(C.Structure):
+    _fields_ = [("bandwidth", C.c_double), ("focused_time", C.c_double), ("n_ref", C.c_double),
+                ("ka_mean", C.c_double), ("num_tiles", C.c_int32), ("shape", C.c_int32),
+                ("lpf_start", C.c_int32), ("lpf_end", C.c_int32), ("downsample_shape", C.c_int32),
+                ("output_shape", C.c_int32), ("crop_start", C.c_int32), ("crop_end", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every exported entry point
 _SIGS = [
     ("s1g_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
@@ -122,6 +132,23 @@ _SIGS = [
                             C.c_int64]),
     ("s1g_to_cf32", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
     ("s1g_write_tiff", C.c_int, [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("s1g_azimuth_beam_angle", C.c_double, [C.c_int]),
+    ("s1g_doppler_centroid_rate", C.c_double, [C.c_double, C.c_int, C.c_int, C.c_int, C.c_double]),
+    ("s1g_doppler_centroid", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_double,
+                                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p]),
+    ("s1g_tops_frequency_lines", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double]),
+    ("s1g_azimuth_frequency_ufr", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64,
+                                            C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                            C.c_int64]),
+    ("s1g_tops_filter_lines", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_double, C.c_void_p]),
+    ("s1g_azimuth_time_ufr_params", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                              C.c_double, C.c_double, C.c_double, C.c_int,
+                                              C.POINTER(TimeUfrParams)]),
+    ("s1g_azimuth_time_ufr", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
+                                       C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                       C.c_void_p, C.c_int64]),
     ("s1g_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
     ("s1g_force_variant", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("s1g_last_variant", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -233,6 +260,17 @@ def write_tiff(path, img: np.ndarray, flip_rows=True):
     rc = lib().s1g_write_tiff(str(path).encode(), img.ctypes.data, img.shape[0], img.shape[1], int(flip_rows))
     if rc:
         raise S1GError(f"s1g_write_tiff failed ({rc})")
+
+
+def azimuth_beam_angle(address):
+    """L0Packet::get_azimuth_beam_angle (host)."""
+    return lib().s1g_azimuth_beam_angle(int(address))
+
+
+def doppler_centroid_rate(pri_us, first_address, last_address, n_packets, velocity):
+    """get_doppler_centroid_rate (host)."""
+    return lib().s1g_doppler_centroid_rate(float(pri_us), int(first_address), int(last_address), int(n_packets),
+                                           float(velocity))
 
 
 # ------------------------------------------------------------- context
@@ -361,6 +399,57 @@ class Context:
         self._check(lib().s1g_rcmc_phase(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), x.stride(0),
                                          _ptr(slant_ranges_), _ptr(range_freqs), _ptr(factors)), "rcmc_phase")
         return x
+
+    # --------------------------------------------------------------- TOPS
+    def doppler_centroid(self, rc, rate, prf, t_near, t_far, n_blocks=30):
+        """get_doppler_centroid of a range-compressed burst -> float64[n]."""
+        out = self.torch.empty(rc.shape[1], dtype=self.torch.float64, device=rc.device)
+        self._bind_stream()
+        self._check(lib().s1g_doppler_centroid(self.h, _prec_of(rc), rc.shape[0], rc.shape[1], _ptr(rc),
+                                               rc.stride(0), float(rate), float(prf), float(t_near), float(t_far),
+                                               int(n_blocks), _ptr(out)), "doppler_centroid")
+        return out
+
+    def azimuth_frequency_ufr(self, rc, rate, burst_duration, prf, doppler_bandwidth):
+        lines = lib().s1g_tops_frequency_lines(rc.shape[0], float(rate), float(burst_duration), float(prf))
+        if lines < 1:
+            raise S1GError("azimuth_frequency_ufr: no output lines")
+        out = self.torch.empty((lines, rc.shape[1]), dtype=rc.dtype, device=rc.device)
+        self._bind_stream()
+        self._check(lib().s1g_azimuth_frequency_ufr(self.h, _prec_of(rc), rc.shape[0], rc.shape[1], _ptr(rc),
+                                                    rc.stride(0), float(rate), float(burst_duration), float(prf),
+                                                    float(doppler_bandwidth), _ptr(out), out.stride(0)),
+                    "azimuth_frequency_ufr")
+        return out
+
+    def tops_filter_lines(self, x, positions, velocity_norms, az_freqs, slant_ranges_, range_freqs, f_dc, t_corr,
+                          ref_range):
+        """In place on x; returns the FM rate float64[lines][n]."""
+        fm = self.torch.empty(x.shape, dtype=self.torch.float64, device=x.device)
+        self._bind_stream()
+        self._check(lib().s1g_tops_filter_lines(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), x.stride(0),
+                                                _ptr(positions), _ptr(velocity_norms), _ptr(az_freqs),
+                                                _ptr(slant_ranges_), _ptr(range_freqs), _ptr(f_dc), _ptr(t_corr),
+                                                float(ref_range), _ptr(fm)), "tops_filter_lines")
+        return fm
+
+    def azimuth_time_ufr_params(self, f_dc, fm, rate, burst_duration, prf, doppler_bandwidth, swath_number):
+        p = TimeUfrParams()
+        self._bind_stream()
+        self._check(lib().s1g_azimuth_time_ufr_params(self.h, fm.shape[0], fm.shape[1], _ptr(f_dc), _ptr(fm),
+                                                      float(rate), float(burst_duration), float(prf),
+                                                      float(doppler_bandwidth), int(swath_number), C.byref(p)),
+                    "azimuth_time_ufr_params")
+        return p.as_dict()
+
+    def azimuth_time_ufr(self, x, f_dc, fm, rate, burst_duration, prf, doppler_bandwidth, swath_number):
+        p = self.azimuth_time_ufr_params(f_dc, fm, rate, burst_duration, prf, doppler_bandwidth, swath_number)
+        out = self.torch.empty((p["crop_end"] - p["crop_start"], x.shape[1]), dtype=x.dtype, device=x.device)
+        self._check(lib().s1g_azimuth_time_ufr(self.h, _prec_of(x), x.shape[0], x.shape[1], _ptr(x), x.stride(0),
+                                               _ptr(f_dc), _ptr(fm), float(rate), float(burst_duration), float(prf),
+                                               float(doppler_bandwidth), int(swath_number), _ptr(out),
+                                               out.stride(0)), "azimuth_time_ufr")
+        return out, p
 
     def scale(self, x, mode):
         """Display scaling of a complex image to float (mode: norm_log, norm,

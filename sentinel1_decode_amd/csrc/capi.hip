@@ -88,6 +88,9 @@ struct s1g_ctx {
     size_t blue_scratch_bytes = 0;
     std::map<std::tuple<int, int, int>, void*> blue_filters;  // (n, prec, inverse)
     double* dsp_scal = nullptr;
+    // buffers of the entry points defined in other files (ctx_aux)
+    void* aux[8] = {};
+    size_t aux_bytes[8] = {};
     // autotuner
     bool autotune = true;
     int force[S1G_NSTAGES] = {-1, -1, -1, -1, -1, -1, -1, -1};
@@ -475,6 +478,13 @@ int ctx_scratch(s1g_ctx* c, size_t bytes, void** p) {
     *p = rc ? nullptr : c->dft_ws;
     return rc;
 }
+int ctx_aux(s1g_ctx* c, int slot, size_t bytes, void** p) {
+    *p = nullptr;
+    if (slot < 0 || slot >= 8) return fail(c, S1G_EARG, "bad buffer slot%s");
+    int rc = ensure(c, &c->aux[slot], &c->aux_bytes[slot], bytes);
+    if (!rc) *p = c->aux[slot];
+    return rc;
+}
 int ctx_rcmc_variant(s1g_ctx* c, int prec, int n_lines, int n_samples, int apply_rcmc, int fallback) {
     const int nvar = rcmc_variants(prec);
     int v = fallback;
@@ -543,6 +553,7 @@ int s1g_destroy(s1g_ctx* c) {
     hipFree(c->blue_scratch);
     for (auto& kv : c->blue_filters) hipFree(kv.second);
     hipFree(c->dsp_scal);
+    for (int k = 0; k < 8; ++k) hipFree(c->aux[k]);
     if (c->own) hipStreamDestroy(c->own);
     if (c->side) hipStreamDestroy(c->side);
     delete c;
@@ -882,6 +893,14 @@ static int dft_cols(s1g_ctx* c, int prec, int rows, int cols, void* data, int64_
     (void)hipStreamSynchronize(c->stream);
     hipFree(t);
     return rc > 0 ? check_launch(c, rc, "dft") : rc;
+}
+
+extern "C++" {
+namespace s1g {
+int ctx_dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, int inverse, bool normalise) {
+    return dft_rows(c, prec, rows, n, data, ld, inverse, normalise);
+}
+}  // namespace s1g
 }
 
 extern "C" int s1g_dft(s1g_ctx* c, int prec, int n_rows, int n_cols, void* data, int64_t ld, int axis, int inverse) {
