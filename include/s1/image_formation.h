@@ -1,8 +1,8 @@
 /*
  * image_formation.h -- drop-in image-formation functions of the reference's
  * image_formation.h (lines 14-69) used on the L0 -> SLC path, on the GPU
- * (FP64).  The TOPS resampling (azimuth_frequency_ufr, azimuth_time_ufr) is
- * not provided.
+ * (FP64), including the TOPS unfolding (azimuth_frequency_ufr,
+ * azimuth_time_ufr).
  */
 #pragma once
 
@@ -39,3 +39,28 @@ D_VEC_1D apply_secondary_range_compression(CF_VEC_1D& range_line, const D_VEC_1D
  * (image_formation.cpp:458-476). */
 void apply_range_cell_migration_correction(CF_VEC_1D& range_line, const D_VEC_1D& slant_ranges,
                                            const D_VEC_1D& range_freqs, const D_VEC_1D& rcmc_factors);
+
+/* floor(num_replicas) copies of the signal followed by the first
+ * floor(frac(num_replicas) * size) samples, in floor(num_replicas * size)
+ * samples (get_tiled_signal, image_formation.cpp:87-119). */
+CF_VEC_1D get_tiled_signal(CF_VEC_1D& signal, const double& num_replicas);
+
+/* TOPS azimuth-frequency unfolding of a range-compressed burst
+ * (azimuth_frequency_ufr, image_formation.cpp:122-235):
+ * floor(|dc_rate T / prf| lines) output lines.  The reference transposes
+ * and transforms its argument in place on the way; here the argument is left
+ * as it was.  dc_estimates and initial_packet are unused, as there. */
+CF_VEC_2D azimuth_frequency_ufr(CF_VEC_2D& range_compressed, D_VEC_1D& dc_estimates, L0Packet& initial_packet,
+                                const double& dc_rate, const double& burst_duration, const double& prf,
+                                const double& doppler_bandwidth);
+
+/* TOPS azimuth-time unfolding of an azimuth-compressed burst
+ * (azimuth_time_ufr, image_formation.cpp:238-388) with the azimuth FM rate
+ * az_fm_rate[line][sample]: floor(prf T) / 2 * 2 output lines.  The
+ * arguments are left as they were (the reference transposes two of them).
+ * Where the reference's reference-point index leaves the FM-rate array (for
+ * bursts with more than twice as many samples as lines) the FM rate at the
+ * middle line and sample is used. */
+CF_VEC_2D azimuth_time_ufr(CF_VEC_2D& azimuth_compressed, D_VEC_1D& dc_estimates, D_VEC_2D& az_fm_rate,
+                           L0Packet& initial_packet, const double& dc_rate, const double& burst_duration,
+                           const double& prf, const double& doppler_bandwidth, const int& swath_number);

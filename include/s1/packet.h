@@ -60,6 +60,9 @@ class L0Packet {
      * 0-3 / 4-7 (packet.cpp:104-122); other codes throw std::runtime_error. */
     [[nodiscard]] char get_rx_polarization() const;
     [[nodiscard]] char get_tx_polarization() const;
+    /* linspace(-0.018, 0.018, 1024)[azimuth_beam_address], radians
+     * (packet.cpp:83-90). */
+    [[nodiscard]] double get_azimuth_beam_angle() const;
     /* Two-way times of the range samples: linspace from rank * PRI + SWST +
      * the suppressed-data time over 2 * num_quads / fs (packet.cpp:304-318). */
     [[nodiscard]] D_VEC_1D get_slant_range_times(int num_ranges = 0) const;

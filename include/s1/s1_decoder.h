@@ -6,13 +6,15 @@
  * and _azimuth_compress (s1_decoder.cpp:448-701) as free functions over one
  * burst's packets.
  *
- * Focusing uses the range-Doppler chain (the reference's non-TOPS formulas:
- * decode, ECCM, range compression, azimuth FFT, RCMC + matched filter,
- * azimuth IFFT).  For IW bursts the reference runs its TOPS variant
- * (Doppler-centroid estimation and azimuth resampling), which is not
- * provided: here the Doppler centroid is zero, as in its non-TOPS branch,
- * and the effective velocity per range sample comes from the state vectors
- * interpolated at the burst's middle row.
+ * Focusing follows the reference's two branches: bursts
+ * (get_azimuth_compressed_burst) and IW swaths take the TOPS branch --
+ * Doppler-centroid estimation, azimuth-frequency unfolding, the per-line
+ * filter with a platform state per line, azimuth IFFT, azimuth-time
+ * unfolding (tops_compress_burst); stripmap swaths take the range-Doppler
+ * chain (decode, ECCM, range compression, azimuth FFT, RCMC + matched
+ * filter, azimuth IFFT) with a zero Doppler centroid and the effective
+ * velocity per range sample from the state vectors at the burst's middle
+ * row.
  */
 #pragma once
 
@@ -36,6 +38,17 @@ CF_VEC_2D range_compress_burst(PACKET_VEC_1D& packets, const bool& do_azimuth_ff
  * (S1_Decoder::_azimuth_compress, s1_decoder.cpp:511-701). */
 CF_VEC_2D azimuth_compress_burst(PACKET_VEC_1D& packets, const D_VEC_1D& effective_velocities,
                                  const D_VEC_1D& doppler_centroids, const S1_ECCM_THRESHOLDS* eccm = nullptr);
+
+/* The TOPS branch of S1_Decoder::_azimuth_compress (s1_decoder.cpp:511-701)
+ * on the GPU: decode (+ ECCM) and range compression, the Doppler-centroid
+ * rate from the first state vector's speed and the burst's beam angles, the
+ * Doppler centroid over 30 range blocks, azimuth-frequency unfolding, the
+ * per-line filter with the platform state interpolated at each unfolded
+ * line's time, azimuth IFFT and azimuth-time unfolding.  std::runtime_error
+ * without state vectors; std::invalid_argument when the burst's geometry
+ * leaves the unfolding's windows (e.g. a burst without beam steering). */
+CF_VEC_2D tops_compress_burst(PACKET_VEC_1D& packets, const STATE_VECTORS& state_vectors,
+                              const S1_ECCM_THRESHOLDS* eccm = nullptr);
 
 class S1_Decoder {
    public:
@@ -78,5 +91,5 @@ class S1_Decoder {
 
     void _validate_request(const std::string& swath, const int& burst = -1);
     CF_VEC_2D _range_compress(PACKET_VEC_1D& packets, const bool& do_azimuth_fft = false);
-    CF_VEC_2D _azimuth_compress(PACKET_VEC_1D& packets);
+    CF_VEC_2D _azimuth_compress(PACKET_VEC_1D& packets, const bool& tops_mode = false);
 };

@@ -30,6 +30,20 @@ CF_VEC_2D compute_axis_dft(CF_VEC_2D& signals, int fft_size = 0, const int& axis
 void compute_axis_dft_in_place(CF_VEC_2D& signals, int fft_size = 0, const int& axis = 0,
                                const bool& inverse = false);
 
+/* Resampling and fitting on the host, as the reference writes them
+ * (signal_processing.cpp:32-233,501-513): linear_resample walks a running
+ * abscissa over the input segments and ends on the last input;
+ * quadratic_resample is three-point Lagrange (its first output interpolates
+ * through a repeated point, 0 / 0, when the lengths differ); polyfit is the
+ * degree-2 least-squares fit by normal equations, coefficients of 1, x, x^2;
+ * fftfreq is numpy's. */
+D_VEC_1D linear_resample(const D_VEC_1D& arr, const int& num_output_samples);
+CF_VEC_1D linear_resample(const CF_VEC_1D& arr, const int& num_output_samples);
+CF_VEC_1D quadratic_resample(const CF_VEC_1D& arr, const int& num_output_samples);
+D_VEC_1D polyfit(const D_VEC_1D& x, const D_VEC_1D& y);
+double polyval(const D_VEC_1D& p, const double& x);
+D_VEC_1D fftfreq(int n, double d = 1.0);
+
 /* std::rotate by size/2 (signal_processing.cpp:713-727). */
 void fftshift_in_place(CF_VEC_1D& signal);
 void fftshift_in_place(CF_VEC_2D& signals);

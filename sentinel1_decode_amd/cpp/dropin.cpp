@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "../../include/s1/doppler.h"
 #include "../../include/s1/image_formation.h"
 #include "../../include/s1/image_write.h"
 #include "../../include/s1/s1_decoder.h"
@@ -252,12 +253,24 @@ char L0Packet::get_tx_polarization() const {
     throw std::runtime_error("The polarization code is invalid.");
 }
 
+double L0Packet::get_azimuth_beam_angle() const {
+    return s1g_azimuth_beam_angle((int)_info.secondary[36]);  // azimuth_beam_address
+}
+
+namespace {
+// The two-way time span of a packet's range samples (packet.cpp:304-318).
+void slant_range_span(const s1g_packet_info& in, double* near, double* far) {
+    const double start = in.swst_us * 1e-6, pri = in.pri_us * 1e-6, rank = (double)in.secondary[26];
+    const double fs = in.fs_mhz * 1e+6, n_rng = 2.0 * in.num_quads;
+    *near = rank * pri + start + (320 / (8 * 37.53472224)) * 1e-6;
+    *far = *near + (n_rng / fs);
+}
+}  // namespace
+
 D_VEC_1D L0Packet::get_slant_range_times(int num_ranges) const {
     if (num_ranges <= 0) num_ranges = 2 * _info.num_quads;
-    const double start = _info.swst_us * 1e-6, pri = _info.pri_us * 1e-6, rank = (double)_info.secondary[26];
-    const double fs = _info.fs_mhz * 1e+6, n_rng = 2.0 * _info.num_quads;
-    const double near = rank * pri + start + (320 / (8 * 37.53472224)) * 1e-6;
-    const double far = near + (n_rng / fs);
+    double near, far;
+    slant_range_span(_info, &near, &far);
     return linspace(near, far, num_ranges);
 }
 
@@ -340,6 +353,106 @@ D_VEC_1D linspace(const double& start, const double& end, const int& size) {
         for (int i = 0; i < size; ++i) out[i] = start + i * step;
     }
     return out;
+}
+
+namespace {
+template <typename V>
+std::vector<V> linear_resample_t(const std::vector<V>& arr, int n_out) {
+    const int n_in = (int)arr.size();
+    if (n_in == n_out) return arr;
+    if (n_in < 2 || n_out < 2) throw std::invalid_argument("linear_resample needs at least 2 points");
+    std::vector<V> out((size_t)n_out);
+    double x = 0.0;
+    const double dx_in = 1.0 / (n_in - 1), dx_out = 1.0 / (n_out - 1);
+    int index = 0;
+    for (int i = 0; i < n_in - 1; i++) {
+        const double x0 = i * dx_in, x1 = (i + 1) * dx_in;
+        const V y0 = arr[i], y1 = arr[i + 1];
+        while (x < x1 && index < n_out) {
+            out[index] = y0 + ((x - x0) * (y1 - y0) / (x1 - x0));
+            x += dx_out;
+            index += 1;
+        }
+    }
+    out[n_out - 1] = arr[n_in - 1];
+    return out;
+}
+}  // namespace
+
+D_VEC_1D linear_resample(const D_VEC_1D& arr, const int& num_output_samples) {
+    return linear_resample_t(arr, num_output_samples);
+}
+CF_VEC_1D linear_resample(const CF_VEC_1D& arr, const int& num_output_samples) {
+    return linear_resample_t(arr, num_output_samples);
+}
+
+CF_VEC_1D quadratic_resample(const CF_VEC_1D& arr, const int& num_output_samples) {
+    const int n_in = (int)arr.size(), n_out = num_output_samples;
+    if (n_in == n_out) return arr;
+    if (n_in < 3) throw std::invalid_argument("Need at least 3 points for quadratic interpolation.");
+    CF_VEC_1D out((size_t)std::max(n_out, 0));
+    const double dx_in = 1.0 / (n_in - 1), dx_out = 1.0 / (n_out - 1);
+    for (int j = 0; j < n_out; ++j) {
+        const double x = j * dx_out;
+        const int i = std::min(int(x / dx_in), n_in - 2);
+        const int i0 = std::max(i - 1, 0), i1 = i, i2 = std::min(i + 1, n_in - 1);
+        const double x0 = i0 * dx_in, x1 = i1 * dx_in, x2 = i2 * dx_in;
+        const double l0 = (x - x1) * (x - x2) / ((x0 - x1) * (x0 - x2));
+        const double l1 = (x - x0) * (x - x2) / ((x1 - x0) * (x1 - x2));
+        const double l2 = (x - x0) * (x - x1) / ((x2 - x0) * (x2 - x1));
+        out[j] = l0 * arr[i0] + l1 * arr[i1] + l2 * arr[i2];
+    }
+    return out;
+}
+
+D_VEC_1D polyfit(const D_VEC_1D& x, const D_VEC_1D& y) {
+    if (x.size() != y.size() || x.size() < 3) throw std::invalid_argument("polyfit needs 3 or more (x, y) pairs");
+    double Sx[5] = {0, 0, 0, 0, 0}, Sy[3] = {0, 0, 0};
+    for (size_t i = 0; i < x.size(); ++i) {
+        double xi = 1.0;
+        for (int j = 0; j <= 4; ++j) {
+            if (j <= 2) Sy[j] += xi * y[i];
+            Sx[j] += xi;
+            xi *= x[i];
+        }
+    }
+    double A[3][3] = {{Sx[0], Sx[1], Sx[2]}, {Sx[1], Sx[2], Sx[3]}, {Sx[2], Sx[3], Sx[4]}};
+    double b[3] = {Sy[0], Sy[1], Sy[2]};
+    for (int i = 0; i < 3; ++i) {
+        for (int k = i + 1; k < 3; ++k) {
+            if (std::abs(A[k][i]) > std::abs(A[i][i])) {
+                for (int j = 0; j < 3; ++j) std::swap(A[i][j], A[k][j]);
+                std::swap(b[i], b[k]);
+            }
+        }
+        for (int k = i + 1; k < 3; ++k) {
+            const double f = A[k][i] / A[i][i];
+            for (int j = i; j < 3; ++j) A[k][j] -= f * A[i][j];
+            b[k] -= f * b[i];
+        }
+    }
+    D_VEC_1D c(3);
+    for (int i = 2; i >= 0; --i) {
+        c[i] = b[i];
+        for (int j = i + 1; j < 3; ++j) c[i] -= A[i][j] * c[j];
+        c[i] /= A[i][i];
+    }
+    return c;
+}
+
+double polyval(const D_VEC_1D& p, const double& x) {
+    double r = 0.0;
+    for (size_t k = p.size(); k-- > 0;) r = r * x + p[k];
+    return r;
+}
+
+D_VEC_1D fftfreq(int n, double d) {
+    D_VEC_1D f((size_t)std::max(n, 0));
+    const double val = 1.0 / (n * d);
+    const int N = (n - 1) / 2 + 1;
+    for (int i = 0; i < N; ++i) f[i] = i * val;
+    for (int i = N; i < n; ++i) f[i] = (i - n) * val;
+    return f;
 }
 
 std::complex<double> sinc_interpolate(const CF_VEC_1D& signal, double t, int L) {
@@ -811,6 +924,205 @@ void apply_range_cell_migration_correction(CF_VEC_1D& range_line, const D_VEC_1D
     download(range_line.data(), dx.p, sizeof(cdouble) * n);
 }
 
+// ------------------------------------------------------------ TOPS (IW)
+namespace {
+// [rows][cols] host rows of equal length -> dense device array.
+int check_rect(const CF_VEC_2D& a, const char* what) {
+    if (a.empty()) throw std::invalid_argument(std::string(what) + ": empty burst");
+    for (const auto& r : a)
+        if (r.size() != a[0].size()) throw std::invalid_argument(std::string(what) + ": rows differ in length");
+    return (int)a[0].size();
+}
+CF_VEC_2D download_burst(const DevBuf& d, int rows, int cols) {
+    CF_VEC_2D out((size_t)rows, CF_VEC_1D((size_t)cols));
+    download_rows(out, cols, d.as<cdouble>());
+    return out;
+}
+}  // namespace
+
+double get_doppler_centroid_rate(PACKET_VEC_1D& burst_packets, const double& velocity) {
+    if (burst_packets.empty()) throw std::invalid_argument("get_doppler_centroid_rate: no packets");
+    const int first = (int)burst_packets.front().info().secondary[36];
+    const int last = (int)burst_packets.back().info().secondary[36];
+    return s1g_doppler_centroid_rate(burst_packets[0].get_pri(), first, last, (int)burst_packets.size(), velocity);
+}
+
+D_VEC_1D get_doppler_centroid(CF_VEC_2D& range_compressed, const double& doppler_centroid_rate,
+                              const double& /*burst_duration*/, L0Packet& first_packet, const int& num_rng_blocks) {
+    const int n = check_rect(range_compressed, "get_doppler_centroid");
+    const int L = (int)range_compressed.size();
+    double near, far;
+    slant_range_span(first_packet.info(), &near, &far);
+    const double prf = 1.0 / (first_packet.get_pri() * 1e-6);
+    DevBuf d(sizeof(cdouble) * (size_t)L * n), dc(8 * (size_t)n);
+    upload_rows(range_compressed, n, d.as<cdouble>());
+    check(s1g_doppler_centroid(ctx(), S1G_F64, L, n, d.p, n, doppler_centroid_rate, prf, near, far, num_rng_blocks,
+                               dc.as<double>()),
+          "Doppler centroid");
+    D_VEC_1D out((size_t)n);
+    download(out.data(), dc.p, 8 * (size_t)n);
+    return out;
+}
+
+CF_VEC_1D get_tiled_signal(CF_VEC_1D& signal, const double& num_replicas) {
+    const int n = (int)signal.size();
+    const int shape = (int)std::floor(num_replicas * n);
+    const int offset = (int)std::floor(num_replicas);
+    CF_VEC_1D out((size_t)std::max(shape, 0));
+    size_t k = 0;
+    for (int i = 0; i < offset; ++i)
+        for (int j = 0; j < n && k < out.size(); ++j) out[k++] = signal[j];
+    if (std::floor(num_replicas) != num_replicas) {
+        const int rest = (int)std::floor((num_replicas - std::floor(num_replicas)) * n);
+        for (int j = 0; j < rest && k < out.size(); ++j) out[k++] = signal[j];
+    }
+    return out;
+}
+
+CF_VEC_2D azimuth_frequency_ufr(CF_VEC_2D& range_compressed, D_VEC_1D& /*dc_estimates*/,
+                                L0Packet& /*initial_packet*/, const double& dc_rate, const double& burst_duration,
+                                const double& prf, const double& doppler_bandwidth) {
+    const int n = check_rect(range_compressed, "azimuth_frequency_ufr");
+    const int L = (int)range_compressed.size();
+    const int rows = s1g_tops_frequency_lines(L, dc_rate, burst_duration, prf);
+    if (rows < 1) throw std::invalid_argument("azimuth_frequency_ufr: no output lines");
+    DevBuf d(sizeof(cdouble) * (size_t)L * n), o(sizeof(cdouble) * (size_t)rows * n);
+    upload_rows(range_compressed, n, d.as<cdouble>());
+    check(s1g_azimuth_frequency_ufr(ctx(), S1G_F64, L, n, d.p, n, dc_rate, burst_duration, prf, doppler_bandwidth,
+                                    o.p, n),
+          "azimuth frequency UFR");
+    return download_burst(o, rows, n);
+}
+
+CF_VEC_2D azimuth_time_ufr(CF_VEC_2D& azimuth_compressed, D_VEC_1D& dc_estimates, D_VEC_2D& az_fm_rate,
+                           L0Packet& /*initial_packet*/, const double& dc_rate, const double& burst_duration,
+                           const double& prf, const double& doppler_bandwidth, const int& swath_number) {
+    const int n = check_rect(azimuth_compressed, "azimuth_time_ufr");
+    const int L = (int)azimuth_compressed.size();
+    if ((int)dc_estimates.size() < n || (int)az_fm_rate.size() != L)
+        throw std::invalid_argument("azimuth_time_ufr: Doppler centroids or FM rates do not match the burst");
+    std::vector<double> fm((size_t)L * n);
+    for (int i = 0; i < L; ++i) {
+        if ((int)az_fm_rate[i].size() < n) throw std::invalid_argument("azimuth_time_ufr: short FM-rate line");
+        std::copy(az_fm_rate[i].begin(), az_fm_rate[i].begin() + n, fm.begin() + (size_t)i * n);
+    }
+    s1g_ctx* c = ctx();
+    DevBuf d(sizeof(cdouble) * (size_t)L * n), dfm(8 * fm.size()), ddc(8 * (size_t)n);
+    upload_rows(azimuth_compressed, n, d.as<cdouble>());
+    upload(dfm.p, fm.data(), 8 * fm.size());
+    upload(ddc.p, dc_estimates.data(), 8 * (size_t)n);
+    s1g_time_ufr_params p;
+    check(s1g_azimuth_time_ufr_params(c, L, n, ddc.as<double>(), dfm.as<double>(), dc_rate, burst_duration, prf,
+                                      doppler_bandwidth, swath_number, &p),
+          "azimuth time UFR");
+    const int rows = p.crop_end - p.crop_start;
+    DevBuf o(sizeof(cdouble) * (size_t)rows * n);
+    check(s1g_azimuth_time_ufr(c, S1G_F64, L, n, d.p, n, ddc.as<double>(), dfm.as<double>(), dc_rate,
+                               burst_duration, prf, doppler_bandwidth, swath_number, o.p, n),
+          "azimuth time UFR");
+    return download_burst(o, rows, n);
+}
+
+CF_VEC_2D tops_compress_burst(PACKET_VEC_1D& packets, const STATE_VECTORS& state_vectors,
+                              const S1_ECCM_THRESHOLDS* eccm_thr) {
+    if (state_vectors.size() == 0) throw std::runtime_error("TOPS focusing needs the file's state vectors");
+    const DeviceBurst b = pack(packets);
+    const int L = b.L, N = b.N;
+    if (eccm_thr) (void)packets[0].get_rx_polarization();
+    const int prec = t_burst_prec;
+    const size_t es = prec == S1G_F32 ? 8 : 16;
+    const L0Packet& first = packets[0];
+    const s1g_packet_info& in = first.info();
+    s1g_ctx* c = ctx();
+    // decode (+ ECCM) and range compression, kept on the device
+    DevBuf d_blob(b.blob.size()), d_offs(8 * (size_t)L), d_lens(4 * (size_t)L), d_nq(4 * (size_t)L),
+        d_st(4 * (size_t)L);
+    upload(d_blob.p, b.blob.data(), b.blob.size());
+    upload(d_offs.p, b.offs.data(), 8 * (size_t)L);
+    upload(d_lens.p, b.lens.data(), 4 * (size_t)L);
+    upload(d_nq.p, b.nq.data(), 4 * (size_t)L);
+    DevBuf d_rc(es * (size_t)L * N);
+    {
+        DevBuf d_dec(es * (size_t)L * N);
+        s1g_range_desc rd{L, N, in.txpsf_mhz, in.txprr_mhz_us, in.pulse_length_us, in.fs_mhz};
+        s1g_eccm_desc ed{L, N, 128, 64, eccm_thr ? eccm_thr->detection : 0.0,
+                         eccm_thr ? eccm_thr->mitigation : 0.0};
+        int rc = s1g_decode_fdbaq(c, d_blob.as<uint8_t>(), d_offs.as<uint64_t>(), d_lens.as<uint32_t>(),
+                                  d_nq.as<int32_t>(), L, b.max_bytes, b.max_quads, prec, d_dec.p, N,
+                                  d_st.as<int32_t>());
+        if (rc == S1G_OK && eccm_thr) rc = s1g_eccm(c, &ed, prec, d_dec.p, N, nullptr);
+        if (rc == S1G_OK) rc = s1g_range_compress(c, &rd, prec, d_dec.p, N, d_rc.p, N);
+        if (rc == S1G_OK) rc = s1g_synchronize(c);
+        check(rc, "range compression");
+        throw_failed_packets(packets, d_st.as<int32_t>());
+    }
+    // burst geometry (s1_decoder.cpp:525-570)
+    const D_VEC_1D& v0 = state_vectors.velocities[0];
+    const double v_norm = std::sqrt(std::pow(v0[0], 2.0) + std::pow(v0[1], 2.0) + std::pow(v0[2], 2.0));
+    const double prf = 1 / (in.pri_us * 1e-6);
+    const double T = double(L) / prf;
+    const int swath = in.swath_number;
+    const double bw = swath == 11 ? 675 : 600;
+    const double t0 = first.get_time();
+    double time_delta = T / prf;
+    const double fs = in.fs_mhz * 1e+6;
+    const double rate = get_doppler_centroid_rate(packets, v_norm);
+    double near, far;
+    slant_range_span(in, &near, &far);
+    DevBuf d_dc(8 * (size_t)N);
+    check(s1g_doppler_centroid(c, prec, L, N, d_rc.p, N, rate, prf, near, far, 30, d_dc.as<double>()),
+          "Doppler centroid");
+    const int rows = s1g_tops_frequency_lines(L, rate, T, prf);
+    if (rows < 3) throw std::invalid_argument("TOPS focusing: the burst unfolds to fewer than 3 lines");
+    DevBuf d_u(es * (size_t)rows * N);
+    check(s1g_azimuth_frequency_ufr(c, prec, L, N, d_rc.p, N, rate, T, prf, bw, d_u.p, N), "azimuth frequency UFR");
+    // a platform state per unfolded line (s1_decoder.cpp:573-591)
+    time_delta /= double(rows) / double(L);
+    std::vector<double> pos(3 * (size_t)rows), vn((size_t)rows);
+    for (int i = 0; i < rows; ++i) {
+        const STATE_VECTOR sv = state_vectors.interpolate(t0 + time_delta * i);
+        const D_VEC_1D& v = sv.velocity;
+        for (int k = 0; k < 3; ++k) pos[3 * (size_t)i + k] = sv.position[k];
+        vn[i] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    const double nr = std::abs(rate * T / prf);
+    const D_VEC_1D az = linspace(-nr * prf / 2, nr * prf / 2, rows);
+    const D_VEC_1D R = first.get_slant_ranges(), tc = first.get_timing_corrections();
+    const D_VEC_1D fr = linspace(-fs / 2, fs / 2, N);
+    DevBuf d_pos(8 * pos.size()), d_vn(8 * vn.size()), d_az(8 * (size_t)rows), d_R(8 * (size_t)N),
+        d_fr(8 * (size_t)N), d_tc(8 * (size_t)N), d_fm(8 * (size_t)rows * N);
+    upload(d_pos.p, pos.data(), 8 * pos.size());
+    upload(d_vn.p, vn.data(), 8 * vn.size());
+    upload(d_az.p, az.data(), 8 * (size_t)rows);
+    upload(d_R.p, R.data(), 8 * (size_t)N);
+    upload(d_fr.p, fr.data(), 8 * (size_t)N);
+    upload(d_tc.p, tc.data(), 8 * (size_t)N);
+    check(s1g_tops_filter_lines(c, prec, rows, N, d_u.p, N, d_pos.as<double>(), d_vn.as<double>(), d_az.as<double>(),
+                                d_R.as<double>(), d_fr.as<double>(), d_dc.as<double>(), d_tc.as<double>(), R[N / 2],
+                                d_fm.as<double>()),
+          "TOPS line filter");
+    check(s1g_dft(c, prec, rows, N, d_u.p, N, 0, 1), "azimuth IFFT");
+    s1g_time_ufr_params p;
+    check(s1g_azimuth_time_ufr_params(c, rows, N, d_dc.as<double>(), d_fm.as<double>(), rate, T, prf, bw, swath, &p),
+          "azimuth time UFR");
+    const int out_rows = p.crop_end - p.crop_start;
+    DevBuf d_out(es * (size_t)out_rows * N);
+    check(s1g_azimuth_time_ufr(c, prec, rows, N, d_u.p, N, d_dc.as<double>(), d_fm.as<double>(), rate, T, prf, bw,
+                               swath, d_out.p, N),
+          "azimuth time UFR");
+    check(s1g_synchronize(c), "TOPS focusing");
+    CF_VEC_2D out((size_t)out_rows, CF_VEC_1D((size_t)N));
+    if (prec == S1G_F64) {
+        download_rows(out, N, d_out.as<cdouble>());
+    } else {
+        std::vector<std::complex<float>> h((size_t)out_rows * N);
+        download(h.data(), d_out.p, es * h.size());
+        for (int i = 0; i < out_rows; ++i)
+            for (int j = 0; j < N; ++j) out[i][j] = cdouble(h[(size_t)i * N + j]);
+    }
+    return out;
+}
+
 // ------------------------------------------------------------- FFT plans
 namespace s1 {
 struct fft_plan_rec {
@@ -959,8 +1271,9 @@ D_VEC_1D S1_Decoder::burst_effective_velocities(PACKET_VEC_1D& packets) {
     return get_effective_velocities(s.position, v, R);
 }
 
-CF_VEC_2D S1_Decoder::_azimuth_compress(PACKET_VEC_1D& packets) {
+CF_VEC_2D S1_Decoder::_azimuth_compress(PACKET_VEC_1D& packets, const bool& tops_mode) {
     S1_ECCM_THRESHOLDS thr{(double)_eccm_detection_threshold, (double)_eccm_mitigation_threshold};
+    if (tops_mode) return tops_compress_burst(packets, _state_vectors, _use_eccm ? &thr : nullptr);
     const D_VEC_1D v = burst_effective_velocities(packets);
     return azimuth_compress_burst(packets, v, D_VEC_1D{0.0}, _use_eccm ? &thr : nullptr);
 }
@@ -975,7 +1288,7 @@ CF_VEC_2D S1_Decoder::get_range_compressed_burst(const std::string& swath, const
 CF_VEC_2D S1_Decoder::get_azimuth_compressed_burst(const std::string& swath, const int& burst) {
     _validate_request(swath, burst);
     PACKET_VEC_1D packets = _echo_packets.at(swath).at((size_t)burst);
-    return _azimuth_compress(packets);
+    return _azimuth_compress(packets, true);  // as s1_decoder.cpp:303, whatever the mode
 }
 
 namespace {
@@ -1019,7 +1332,7 @@ CF_VEC_2D S1_Decoder::get_azimuth_compressed_swath(const std::string& swath) {
         for (auto& burst : _echo_packets.at(swath)) {
             if (burst.size() < 1000) continue;  // s1_decoder.cpp:322-330
             PACKET_VEC_1D packets = burst;
-            append_rows(out, _azimuth_compress(packets), 0);
+            append_rows(out, _azimuth_compress(packets, true), 0);
             pad20();
         }
     } else if (kStripmapSwaths.count(num)) {

@@ -826,7 +826,10 @@ int s1g_eccm(s1g_ctx* c, const s1g_eccm_desc* d, int prec, void* data, int64_t l
 
 // ------------------------------------------------------------ general DSP
 // Rows x n DFT along rows, in place: Stockham passes when n factors over
-// the pass radices, else Bluestein through a power-of-two length.
+// the pass radices, else Bluestein through a power-of-two length.  Rows go
+// in batches that keep the work arrays within kDftWorkBudget bytes.
+constexpr size_t kDftWorkBudget = size_t(4) << 30;
+
 static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, int inverse, bool normalise = true) {
     if (rows <= 0 || n <= 0) return 0;
     const DevFft* f;
@@ -835,12 +838,17 @@ static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t l
     const size_t es = esize(prec);
     const double scale = inverse && normalise ? 1.0 / (double)n : 1.0;
     if (f->plan.npass >= 0) {
-        rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 2 * es * (size_t)rows * ld);
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(rows, kDftWorkBudget / (2 * es * (size_t)ld)));
+        rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 2 * es * (size_t)chunk * ld);
         if (rc) return rc;
         char* w1 = (char*)c->dft_ws;
-        char* w2 = w1 + es * (size_t)rows * ld;
-        rc = launch_fft_global(c->stream, prec, *f, data, w1, w2, rows, ld, inverse);
-        if (!rc) rc = launch_copy_scale(c->stream, prec, w1, ld, data, ld, rows, n, scale);
+        char* w2 = w1 + es * (size_t)chunk * ld;
+        for (int r0 = 0; r0 < rows && !rc; r0 += chunk) {
+            const int nr = std::min(chunk, rows - r0);
+            char* x = (char*)data + es * (size_t)ld * r0;
+            rc = launch_fft_global(c->stream, prec, *f, x, w1, w2, nr, ld, inverse);
+            if (!rc) rc = launch_copy_scale(c->stream, prec, w1, ld, x, ld, nr, n, scale);
+        }
         return check_launch(c, rc, "dft");
     }
     int M = 1;
@@ -864,13 +872,18 @@ static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t l
         }
         c->blue_filters[key] = bhat;
     }
-    rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 3 * es * (size_t)rows * M);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(rows, kDftWorkBudget / (3 * es * (size_t)M)));
+    rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 3 * es * (size_t)chunk * M);
     if (rc) return rc;
     char* w1 = (char*)c->dft_ws;
-    char* w2 = w1 + es * (size_t)rows * M;
-    char* w3 = w2 + es * (size_t)rows * M;
-    return check_launch(c, launch_bluestein(c->stream, prec, *fM, bhat, data, ld, rows, n, inverse, scale, w1, w2, w3),
-                        "dft");
+    char* w2 = w1 + es * (size_t)chunk * M;
+    char* w3 = w2 + es * (size_t)chunk * M;
+    for (int r0 = 0; r0 < rows && !rc; r0 += chunk) {
+        const int nr = std::min(chunk, rows - r0);
+        rc = launch_bluestein(c->stream, prec, *fM, bhat, (char*)data + es * (size_t)ld * r0, ld, nr, n, inverse,
+                              scale, w1, w2, w3);
+    }
+    return check_launch(c, rc, "dft");
 }
 
 // Along columns: the LDS column kernel when the length has a plan, else a

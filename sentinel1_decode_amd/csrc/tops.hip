@@ -31,9 +31,6 @@ constexpr double kWavelength = kC / kF0;
 // Slots of the context's aux buffers used here.
 enum { kBufA = 0, kBufB = 1, kBufC = 2, kTables = 3, kFmT = 4, kOut = 5, kVrel = 6, kFactors = 7 };
 
-// Row DFTs are batched so that Bluestein's three work arrays stay bounded.
-constexpr size_t kDftBudget = size_t(4) << 30;
-
 int status(s1g_ctx* c, const char* what) {
     const hipError_t e = hipGetLastError();
     if (e == hipSuccess) return S1G_OK;
@@ -42,20 +39,9 @@ int status(s1g_ctx* c, const char* what) {
     return ctx_error(c, S1G_EHIP, buf);
 }
 
-size_t esize(int prec) { return prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>); }
-
+// Row DFTs (ctx_dft_rows batches rows so that the work arrays stay bounded).
 int dft_rows_batched(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, int inverse, bool normalise) {
-    int M = 1;
-    while (M < 2 * n - 1) M <<= 1;
-    const size_t per_row = std::max<size_t>(3 * esize(prec) * (size_t)M, 2 * esize(prec) * (size_t)ld);
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)rows, kDftBudget / per_row));
-    for (int r0 = 0; r0 < rows; r0 += chunk) {
-        const int nr = std::min(chunk, rows - r0);
-        char* p = (char*)data + esize(prec) * (size_t)ld * r0;
-        int rc = ctx_dft_rows(c, prec, nr, n, p, ld, inverse, normalise);
-        if (rc) return rc;
-    }
-    return S1G_OK;
+    return ctx_dft_rows(c, prec, rows, n, data, ld, inverse, normalise);
 }
 
 // linspace (signal_processing.cpp:17-28): start + i step.

@@ -333,16 +333,27 @@ static void level0_file(const std::string& dir) {
     report("S1_Decoder effective velocities", verr < 1e-14, verr);
     const CF_VEC_1D r = flat(d.get_range_compressed_burst("IW2", 1, true));
     report("S1_Decoder::get_range_compressed_burst (range-Doppler)", rel(r, rd) < 1e-11, rel(r, rd));
-    const CF_VEC_1D a = flat(d.get_azimuth_compressed_burst("IW2", 1));
-    report("S1_Decoder::get_azimuth_compressed_burst", rel(a, slc) < 1e-10, rel(a, slc));
+    // the range-Doppler chain of a burst (the reference's non-TOPS branch)
+    const CF_VEC_1D a = flat(azimuth_compress_burst(packets, v, {0.0}));
+    report("azimuth_compress_burst of S1_Decoder's burst", rel(a, slc) < 1e-10, rel(a, slc));
+    // bursts and IW swaths take the TOPS branch
+    const auto tops = read_file<cdouble>(dir + "/l0_tops.bin");
+    const auto tshape = read_file<int64_t>(dir + "/l0_tops_shape.bin");
+    const CF_VEC_2D tb = d.get_azimuth_compressed_burst("IW2", 1);
+    const CF_VEC_1D t = flat(tb);
+    const bool shape_ok = tshape.size() == 2 && (int64_t)tb.size() == tshape[0] && !tb.empty() &&
+                          (int64_t)tb[0].size() == tshape[1];
+    report("S1_Decoder::get_azimuth_compressed_burst (TOPS) shape", shape_ok, (double)tb.size());
+    report("S1_Decoder::get_azimuth_compressed_burst (TOPS) vs oracle", shape_ok && rel(t, tops) < 1e-10,
+           rel(t, tops));
     const CF_VEC_2D sw = d.get_azimuth_compressed_swath("IW2");
-    const size_t n0 = d.bursts("IW2")[0].size(), n1 = d.bursts("IW2")[1].size();
-    bool ok = sw.size() == n0 + 20 + n1 + 20;
+    const size_t m0 = (size_t)tshape[0];
+    bool ok = shape_ok && sw.size() == 2 * (m0 + 20);
     if (ok) {
-        const CF_VEC_2D part(sw.begin() + (long)(n0 + 20), sw.begin() + (long)(n0 + 20 + n1));
-        ok = rel(flat(part), slc) < 1e-10;
+        const CF_VEC_2D part(sw.begin() + (long)(m0 + 20), sw.begin() + (long)(2 * m0 + 20));
+        ok = rel(flat(part), tops) < 1e-10;
     }
-    report("S1_Decoder::get_azimuth_compressed_swath", ok, 0);
+    report("S1_Decoder::get_azimuth_compressed_swath (TOPS bursts + 20 zero lines)", ok, (double)sw.size());
 }
 
 // The reference's image-formation helpers and FFTW-style plans.

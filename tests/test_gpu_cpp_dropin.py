@@ -53,7 +53,7 @@ def _write_level0_case(d):
 
     import torch
     import oracle
-    from chain_ref import oracle_chain
+    from chain_ref import oracle_chain, oracle_tops
     from sentinel1_decode_amd import capi, pipeline, synth
     specs = []
     for k in range(2):
@@ -76,6 +76,10 @@ def _write_level0_case(d):
     np.ascontiguousarray(ref["rd"], np.complex128).tofile(d / "l0_rd.bin")
     np.ascontiguousarray(ref["slc"], np.complex128).tofile(d / "l0_slc.bin")
     np.ascontiguousarray(v, np.float64).tofile(d / "l0_vrel.bin")
+    # the TOPS branch that get_azimuth_compressed_burst takes
+    tops = oracle_tops(ob, F, oracle.state_vectors(oracle.header_fields(s.stream, oracle.index_level0(s.stream))))
+    np.ascontiguousarray(tops["slc"], np.complex128).tofile(d / "l0_tops.bin")
+    np.asarray(tops["slc"].shape, np.int64).tofile(d / "l0_tops_shape.bin")
     counts = [prod.swaths()[n] for n in ("IW1", "IW2", "IW3")] + [prod.n_bursts(n) for n in ("IW1", "IW2", "IW3")]
     np.asarray(counts + [len(prod.state_vectors)], np.int64).tofile(d / "l0_counts.bin")
     ctx.close()
