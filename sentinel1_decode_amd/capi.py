@@ -188,6 +188,14 @@ def parse_packet_header(packet: bytes) -> PacketInfo:
     return info
 
 
+def linspace(start, end, size):
+    """linspace (signal_processing.cpp:17-28): start + i (end - start) / (size - 1)."""
+    if size == 1:
+        return np.array([float(start)])
+    step = (end - start) / (size - 1)
+    return start + np.arange(size, dtype=np.float64) * step
+
+
 def slant_ranges(info: PacketInfo, num_ranges: int = 0) -> np.ndarray:
     n = num_ranges if num_ranges > 0 else 2 * info.num_quads
     out = np.zeros(n)

@@ -89,8 +89,8 @@ struct s1g_ctx {
     std::map<std::tuple<int, int, int>, void*> blue_filters;  // (n, prec, inverse)
     double* dsp_scal = nullptr;
     // buffers of the entry points defined in other files (ctx_aux)
-    void* aux[8] = {};
-    size_t aux_bytes[8] = {};
+    void* aux[16] = {};
+    size_t aux_bytes[16] = {};
     // autotuner
     bool autotune = true;
     int force[S1G_NSTAGES] = {-1, -1, -1, -1, -1, -1, -1, -1};
@@ -480,7 +480,7 @@ int ctx_scratch(s1g_ctx* c, size_t bytes, void** p) {
 }
 int ctx_aux(s1g_ctx* c, int slot, size_t bytes, void** p) {
     *p = nullptr;
-    if (slot < 0 || slot >= 8) return fail(c, S1G_EARG, "bad buffer slot%s");
+    if (slot < 0 || slot >= 16) return fail(c, S1G_EARG, "bad buffer slot%s");
     int rc = ensure(c, &c->aux[slot], &c->aux_bytes[slot], bytes);
     if (!rc) *p = c->aux[slot];
     return rc;
@@ -553,7 +553,7 @@ int s1g_destroy(s1g_ctx* c) {
     hipFree(c->blue_scratch);
     for (auto& kv : c->blue_filters) hipFree(kv.second);
     hipFree(c->dsp_scal);
-    for (int k = 0; k < 8; ++k) hipFree(c->aux[k]);
+    for (int k = 0; k < 16; ++k) hipFree(c->aux[k]);
     if (c->own) hipStreamDestroy(c->own);
     if (c->side) hipStreamDestroy(c->side);
     delete c;

@@ -161,24 +161,32 @@ constexpr double kF0 = 5.405000454334350e9;
 constexpr double kPI = 3.14159265358979;  // the reference's PI (structs.h:23)
 
 // get_effective_velocities (image_formation.cpp:391-419) in its expression
-// order; "lat" is the ratio z / x, as in the reference.
-__global__ void effective_velocity_kernel(const double* __restrict__ pos, const double* __restrict__ vnorm,
-                                          int n_lines, const double* __restrict__ R, int n, double* __restrict__ out,
-                                          int64_t ld) {
+// order; "lat" is the ratio z / x, as in the reference.  The per-line Earth
+// radius and platform distance (cos / sin of lat) come from the host's libm,
+// the per-sample part is IEEE arithmetic without contractions, so v_rel is
+// the reference's to the last bit (the azimuth filter's phase, 4 pi R D /
+// lambda ~ 2e8 rad, turns one ulp of D into ~2e-8 rad).
+void line_radii(double px, double py, double pz, double* re, double* alt) {
 #pragma clang fp contract(off)
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (j >= n || i >= n_lines) return;
-    const double px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-    const double v = vnorm[i];
     const double lat = pz / px;
     const double cl = cos(lat), sl = sin(lat);
     const double t1 = kSemiMajor * kSemiMajor * cl, t2 = kSemiMinor * kSemiMinor * sl;
     const double num = t1 * t1 + t2 * t2;
     const double t3 = kSemiMajor * cl, t4 = kSemiMinor * sl;
     const double den = t3 * t3 + t4 * t4;
-    const double re = sqrt(num / den);
-    const double alt = sqrt(px * px + py * py + pz * pz);
+    *re = sqrt(num / den);
+    *alt = sqrt(px * px + py * py + pz * pz);
+}
+
+__global__ void effective_velocity_kernel(const double* __restrict__ radii, const double* __restrict__ vnorm,
+                                          int n_lines, const double* __restrict__ R, int n, double* __restrict__ out,
+                                          int64_t ld) {
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= n || i >= n_lines) return;
+    const double v = vnorm[i];
+    const double re = radii[2 * i], alt = radii[2 * i + 1];
     const double r = R[j];
     const double beta = (re * re + alt * alt - r * r) / (2 * re * alt);
     const double vg = re * v * beta / alt;
@@ -377,8 +385,20 @@ int s1g_effective_velocities(s1g_ctx* c, const double* positions, const double* 
     if (n_lines < 0 || n_samples < 0 || ld_out < n_samples) return ctx_error(c, S1G_EARG, "bad shape");
     if (!n_lines || !n_samples) return S1G_OK;
     if (!positions || !velocity_norms || !slant_ranges || !out) return ctx_error(c, S1G_EARG, "null pointer");
+    hipStream_t s = ctx_stream(c);
+    std::vector<double> pos(3 * (size_t)n_lines), radii(2 * (size_t)n_lines);
+    if (hipMemcpyAsync(pos.data(), positions, sizeof(double) * pos.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return launch_status(c, "effective velocities");
+    for (int i = 0; i < n_lines; ++i) line_radii(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], &radii[2 * i], &radii[2 * i + 1]);
+    void* d_radii = nullptr;
+    int rc = ctx_aux(c, 8, sizeof(double) * radii.size(), &d_radii);
+    if (rc) return rc;
+    if (hipMemcpyAsync(d_radii, radii.data(), sizeof(double) * radii.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return launch_status(c, "effective velocities");
     dim3 grid((unsigned)((n_samples + 255) / 256), (unsigned)n_lines), block(256);
-    hipLaunchKernelGGL(effective_velocity_kernel, grid, block, 0, ctx_stream(c), positions, velocity_norms, n_lines,
+    hipLaunchKernelGGL(effective_velocity_kernel, grid, block, 0, s, (const double*)d_radii, velocity_norms, n_lines,
                        slant_ranges, n_samples, out, ld_out);
     return launch_status(c, "effective velocities");
 }

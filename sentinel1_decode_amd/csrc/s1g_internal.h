@@ -76,7 +76,7 @@ int ctx_error(s1g_ctx* c, int code, const char* msg);
 // The context's general scratch buffer, grown to at least `bytes` (shared
 // with the DFT entry points; valid until the next call that uses it).
 int ctx_scratch(s1g_ctx* c, size_t bytes, void** p);
-// Buffer `slot` (0..7) of the context, grown to at least `bytes`; kept
+// Buffer `slot` (0..15) of the context, grown to at least `bytes`; kept
 // until the context is destroyed (valid until the next call that grows it).
 int ctx_aux(s1g_ctx* c, int slot, size_t bytes, void** p);
 // In-place DFT of `rows` rows of length n (Stockham, else Bluestein); the

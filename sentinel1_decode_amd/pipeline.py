@@ -301,7 +301,64 @@ class Level0Product:
             self.ctx.azimuth_fft(rc)
         return rc
 
+    def get_tops_burst(self, swath, burst, prec=capi.S1G_F64):
+        """The TOPS branch of S1_Decoder::_azimuth_compress (s1_decoder.cpp:
+        511-701): range compression, the Doppler-centroid rate from the first
+        state vector's speed and the burst's beam angles, the Doppler centroid
+        over 30 range blocks, azimuth-frequency unfolding, the per-line filter
+        with the platform state at each unfolded line's time, azimuth IFFT and
+        azimuth-time unfolding.  Returns the focused burst (device)."""
+        torch = self.torch
+        if not self.state_vectors:
+            raise capi.S1GError("TOPS focusing needs the file's state vectors")
+        idx = self.packet_indices(swath, burst)
+        rc = self.get_range_compressed_burst(swath, burst, prec=prec)
+        L, N = rc.shape
+        first = int(idx[0])
+        info = capi.PacketInfo()
+        row = np.ascontiguousarray(self.fields[first], np.int64)
+        capi.lib().s1g_packet_info_from_fields(row[:8].ctypes.data, row[8:].ctypes.data, C.byref(info))
+        prf = 1 / (info.pri_us * 1e-6)
+        T = L / prf
+        swath_number = int(info.swath_number)
+        bw = 675.0 if swath_number == 11 else 600.0
+        v0 = self.state_vectors[0].velocity
+        v_norm = float(np.sqrt(v0[0] ** 2.0 + v0[1] ** 2.0 + v0[2] ** 2.0))
+        az_col = 8 + 36
+        rate = capi.doppler_centroid_rate(info.pri_us, int(self.fields[first, az_col]),
+                                          int(self.fields[int(idx[-1]), az_col]), L, v_norm)
+        rank = float(info.secondary[26])
+        near = rank * (info.pri_us * 1e-6) + info.swst_us * 1e-6 + (320 / (8 * 37.53472224)) * 1e-6
+        far = near + (2.0 * info.num_quads) / (info.fs_mhz * 1e6)
+        dc = self.ctx.doppler_centroid(rc, rate, prf, near, far, 30)
+        u = self.ctx.azimuth_frequency_ufr(rc, rate, T, prf, bw)
+        rows = u.shape[0]
+        t0 = self.packet_time(first)
+        dt = (T / prf) / (rows / L)
+        pos = np.zeros((rows, 3))
+        vn = np.zeros(rows)
+        for i in range(rows):
+            st = capi.interpolate_state(self.state_vectors, t0 + dt * i)
+            pos[i] = list(st.position)
+            v = st.velocity
+            vn[i] = np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+        geo = geometry_from_info(info, L)
+        nr = abs(rate * T / prf)
+        dev = self.d_stream.device
+        g = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dev)
+        faz = capi.linspace(-nr * prf / 2, nr * prf / 2, rows)
+        fs = info.fs_mhz * 1e6
+        fr = capi.linspace(-fs / 2, fs / 2, N)
+        fm = self.ctx.tops_filter_lines(u, g(pos), g(vn), g(faz), g(geo.slant_ranges), g(fr), dc, g(geo.t_corr),
+                                        float(geo.slant_ranges[N // 2]))
+        self.ctx.dft(u, axis=0, inverse=True)
+        out, _ = self.ctx.azimuth_time_ufr(u, dc, fm, rate, T, prf, bw, swath_number)
+        return out
+
     def get_azimuth_compressed_burst(self, swath, burst, prec=capi.S1G_F64):
+        """The range-Doppler chain of one burst (the hot path; the C++
+        S1_Decoder, like the reference, focuses bursts with the TOPS branch,
+        which get_tops_burst gives here)."""
         db = self.burst(swath, burst)
         ed = self._eccm_desc(db)
         out = db.focus(self.ctx, prec, eccm=ed)
