@@ -10,11 +10,12 @@ oracle (CPU FP64 restatement) is timed on the host cores as the CPU baseline.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
     python bench.py --copy-calibration   # plain copy for HBM counter scaling
-    python bench.py --workload {iw-batch,iw2-swath-eccm,autofocus}   # the other workloads
+    python bench.py --workload {iw-batch,iw2-swath-eccm,autofocus,iw-tops}   # the other workloads
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -254,6 +255,57 @@ def run_iw_batch(steps, warmup):
             "graph_speedup": round(ms_seq / ms_graph, 3), "packets_failed": failed}
 
 
+def run_iw_tops(steps, warmup, lines=1480, samples=24000, cpu=True):
+    """One IW2 burst (lines x samples, 400 Rayleigh targets, clutter 0.1) in a
+    Level-0 stream whose beam sweeps addresses 213..810 (+-0.0105 rad, a
+    TOPS steering rate near IW's) through the TOPS branch: range compression
+    from packets, Doppler centroid, azimuth-frequency unfolding, line
+    filter, azimuth IFFT, azimuth-time unfolding (Level0Product.get_tops_burst)."""
+    import torch
+    from sentinel1_decode_amd import capi, pipeline, synth
+    t0 = time.perf_counter()
+    spec = synth.BurstSpec(n_lines=lines, n_samples=samples, rgdec=8, pri_code=21861, n_targets=400,
+                           amp_mode="rayleigh", swath_number=11, seed=41, clutter_sigma=0.1)
+    s = synth.level0_stream([spec], beam_addresses=(213, 810))
+    print(f"# TOPS burst synthesised in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    ctx = capi.Context(0)
+    prod = pipeline.Level0Product(s.stream, ctx)
+    res = {}
+    for prec, name in ((capi.S1G_F64, "f64"), (capi.S1G_F32, "f32")):
+        out = prod.get_tops_burst("IW2", 0, prec=prec)
+        res[name] = {"ms_per_burst": round(_timed(lambda: prod.get_tops_burst("IW2", 0, prec=prec), steps, warmup), 3),
+                     "out_shape": list(out.shape)}
+        print(f"# TOPS {name}: {res[name]}", file=sys.stderr, flush=True)
+    line = {"metric": "TOPS-focused IW2 burst, ms per burst (packets -> SLC)", "value": res["f64"]["ms_per_burst"],
+            "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": res["f64"]["ms_per_burst"],
+            "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"IW2 burst {lines}x{samples} FDBAQ packets, beam 213..810, TOPS branch"},
+            "f32": res["f32"], "f64": res["f64"]}
+    ctx.close()
+    if cpu:
+        # the oracle's TOPS chain on a bounded sample: the same burst cut to
+        # 1024 range samples (cost is about linear in samples)
+        import oracle
+        sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+        import types
+        from chain_ref import oracle_tops
+        small = dataclasses.replace(spec, n_samples=1024)
+        ss = synth.level0_stream([small], beam_addresses=(213, 810))
+        offs = oracle.index_level0(ss.stream)
+        F = oracle.header_fields(ss.stream, offs)
+        uo = np.asarray(offs, np.uint64) + 68
+        ob = types.SimpleNamespace(stream=ss.stream, ud_offsets=uo, ud_lengths=(F[:, 7] + 1 - 62).astype(np.uint32),
+                                   num_quads=F[:, oracle.SEC["num_quadratures"]].astype(np.int32), spec=small)
+        threads = oracle.set_threads(0)
+        t1 = time.perf_counter()
+        oracle_tops(ob, F, oracle.state_vectors(F))
+        dt = time.perf_counter() - t1
+        line["cpu_baseline"] = {"value": round(dt * 1e3 * samples / 1024, 1), "unit": "ms per burst (scaled)",
+                                "cores": threads, "kind": "port",
+                                "sample": f"oracle TOPS chain on {lines}x1024 in {dt:.2f} s, x {samples}/1024"}
+    return line
+
+
 def run_swath_eccm(steps, warmup, n_bursts=9):
     """An IW2 swath of 9 bursts (1480 x 24000 each) with narrowband RFI on
     20 % of the lines (runs of 10-50 lines, 3 tones within +-28 MHz, INR
@@ -415,7 +467,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain copy of known size (for counter calibration)")
     ap.add_argument("--workload", default="iw2-burst",
-                    choices=["iw2-burst", "iw-batch", "iw2-swath-eccm", "autofocus"],
+                    choices=["iw2-burst", "iw-batch", "iw2-swath-eccm", "autofocus", "iw-tops"],
                     help="iw2-burst (the default line); the others print their own line")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -427,7 +479,9 @@ def main():
             sys.exit(2)
         fn = {"iw-batch": lambda: run_iw_batch(args.steps, args.warmup),
               "iw2-swath-eccm": lambda: run_swath_eccm(args.steps, args.warmup),
-              "autofocus": lambda: run_autofocus()}[args.workload]
+              "autofocus": lambda: run_autofocus(),
+              "iw-tops": lambda: run_iw_tops(args.steps, args.warmup, args.lines, args.samples,
+                                             cpu=not args.no_cpu)}[args.workload]
         print(json.dumps(fn()), flush=True)
         return
     if args.gpus != 1:

@@ -195,6 +195,13 @@ int s1g_state_vectors(const int64_t* fields, int n_packets, s1g_state_vector* ou
  * the reference does it), the first / last vector outside their span. */
 int s1g_interpolate_state(const s1g_state_vector* vectors, int n, double time, s1g_state_vector* out);
 
+/* Host function.  The platform state at t0 + dt * i for i < count (the
+ * per-line states of S1_Decoder::_azimuth_compress, s1_decoder.cpp:
+ * 579-591): positions[3 i .. 3 i + 2] and velocity_norms[i] =
+ * sqrt(vx^2 + vy^2 + vz^2) of s1g_interpolate_state's result. */
+int s1g_line_states(const s1g_state_vector* vectors, int n, double t0, double dt, int count, double* positions,
+                    double* velocity_norms);
+
 /* Effective velocities for n_lines platform states and n_samples slant
  * ranges (get_effective_velocities, image_formation.cpp:391-419):
  * out[i * ld_out + j] = v_rel of line i at range j, from positions[3 i ..

@@ -118,6 +118,8 @@ _SIGS = [
     ("s1g_segment_bursts", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]),
     ("s1g_state_vectors", C.c_int, [C.c_void_p, C.c_int, C.POINTER(StateVector), C.c_int, C.POINTER(C.c_int)]),
     ("s1g_interpolate_state", C.c_int, [C.POINTER(StateVector), C.c_int, C.c_double, C.POINTER(StateVector)]),
+    ("s1g_line_states", C.c_int, [C.POINTER(StateVector), C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                   C.c_void_p]),
     ("s1g_effective_velocities", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
                                            C.c_void_p, C.c_int64]),
     ("s1g_secondary_range_compression", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64,
@@ -256,6 +258,17 @@ def interpolate_state(vectors, time: float) -> StateVector:
     if rc:
         raise S1GError(f"s1g_interpolate_state failed ({rc})")
     return out
+
+
+def line_states(vectors, t0: float, dt: float, count: int):
+    """Positions [count][3] and speeds [count] of the platform at t0 + dt i."""
+    arr = (StateVector * len(vectors))(*vectors)
+    pos = np.zeros((count, 3))
+    vn = np.zeros(count)
+    rc = lib().s1g_line_states(arr, len(vectors), float(t0), float(dt), int(count), pos.ctypes.data, vn.ctypes.data)
+    if rc:
+        raise S1GError(f"s1g_line_states failed ({rc})")
+    return pos, vn
 
 
 SCALE_MODES = {"norm_log": 0, "norm": 1, "mag": 2, "real": 3, "imag": 4}

@@ -1079,11 +1079,11 @@ CF_VEC_2D tops_compress_burst(PACKET_VEC_1D& packets, const STATE_VECTORS& state
     // a platform state per unfolded line (s1_decoder.cpp:573-591)
     time_delta /= double(rows) / double(L);
     std::vector<double> pos(3 * (size_t)rows), vn((size_t)rows);
-    for (int i = 0; i < rows; ++i) {
-        const STATE_VECTOR sv = state_vectors.interpolate(t0 + time_delta * i);
-        const D_VEC_1D& v = sv.velocity;
-        for (int k = 0; k < 3; ++k) pos[3 * (size_t)i + k] = sv.position[k];
-        vn[i] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    {
+        std::vector<s1g_state_vector> sv(state_vectors.size());
+        for (size_t i = 0; i < sv.size(); ++i) sv[i] = from_rows(state_vectors, i);
+        check(s1g_line_states(sv.data(), (int)sv.size(), t0, time_delta, rows, pos.data(), vn.data()),
+              "line states");
     }
     const double nr = std::abs(rate * T / prf);
     const D_VEC_1D az = linspace(-nr * prf / 2, nr * prf / 2, rows);

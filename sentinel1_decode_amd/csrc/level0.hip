@@ -379,6 +379,20 @@ int s1g_interpolate_state(const s1g_state_vector* v, int n, double time, s1g_sta
     return S1G_OK;
 }
 
+int s1g_line_states(const s1g_state_vector* v, int n, double t0, double dt, int count, double* positions,
+                    double* velocity_norms) {
+    if (!v || n <= 0 || count < 0 || (count && (!positions || !velocity_norms))) return S1G_EARG;
+    for (int i = 0; i < count; ++i) {
+        s1g_state_vector st;
+        const int rc = s1g_interpolate_state(v, n, t0 + dt * i, &st);
+        if (rc) return rc;
+        for (int k = 0; k < 3; ++k) positions[3 * (int64_t)i + k] = st.position[k];
+        const double* w = st.velocity;
+        velocity_norms[i] = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    }
+    return S1G_OK;
+}
+
 int s1g_effective_velocities(s1g_ctx* c, const double* positions, const double* velocity_norms, int n_lines,
                              const double* slant_ranges, int n_samples, double* out, int64_t ld_out) {
     if (!c) return S1G_EARG;

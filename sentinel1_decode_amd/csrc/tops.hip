@@ -498,7 +498,14 @@ int freq_ufr_t(s1g_ctx* c, int L, int n, const void* in, int64_t ld_in, double n
     return status(c, "azimuth frequency UFR");
 }
 
-// Copy helpers for the set-up: one double, and column 0 of fm.
+// Sample 0 of every line of fm, gathered into a contiguous array.
+__global__ void gather_col0_kernel(const double* __restrict__ fm, int64_t ld, int n_az, double* __restrict__ out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n_az) out[a] = fm[(int64_t)a * ld];
+}
+
+// The set-up's inputs on the host: sample 0 of fm on every line, f_dc at
+// the middle sample, and the FM rate n_ref divides by.
 int read_params_inputs(s1g_ctx* c, int n_az, int n, const double* dc, const double* fm, std::vector<double>& col0,
                        double* dc_mid, double* fm_ref) {
     hipStream_t s = ctx_stream(c);
@@ -506,12 +513,23 @@ int read_params_inputs(s1g_ctx* c, int n_az, int n, const double* dc, const doub
     // the FM rate n_ref divides by (the reference's transposed index where
     // it lies inside the array)
     int64_t ref = (n_az / 2 < n && n / 2 < n_az) ? (int64_t)(n / 2) * n + n_az / 2 : (int64_t)(n_az / 2) * n + n / 2;
-    if (hipMemcpy2DAsync(col0.data(), sizeof(double), fm, sizeof(double) * (size_t)n, sizeof(double), (size_t)n_az,
-                         hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(dc_mid, dc + n / 2, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(fm_ref, fm + ref, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    void* d_col = nullptr;
+    int rc = ctx_aux(c, 9, sizeof(double) * ((size_t)n_az + 2), &d_col);
+    if (rc) return rc;
+    double* col = (double*)d_col;
+    hipLaunchKernelGGL(gather_col0_kernel, dim3((unsigned)((n_az + 255) / 256)), dim3(256), 0, s, fm, (int64_t)n,
+                       n_az, col);
+    if ((rc = status(c, "azimuth time UFR set-up"))) return rc;
+    if (hipMemcpyAsync(col + n_az, dc + n / 2, sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(col + n_az + 1, fm + ref, sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return status(c, "azimuth time UFR set-up");
+    std::vector<double> h((size_t)n_az + 2);
+    if (hipMemcpyAsync(h.data(), col, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return status(c, "azimuth time UFR set-up");
+    std::copy(h.begin(), h.begin() + n_az, col0.begin());
+    *dc_mid = h[n_az];
+    *fm_ref = h[n_az + 1];
     return S1G_OK;
 }
 
@@ -764,7 +782,16 @@ int s1g_azimuth_time_ufr_params(s1g_ctx* c, int n_lines, int n_samples, const do
     if (rc) return rc;
     const double nr = fabs(rate * burst_duration / prf);
     rc = time_params(col0, n_lines, dc_mid, fm_ref, nr, prf, burst_duration, doppler_bandwidth, swath_number, out);
-    if (rc) return ctx_error(c, S1G_EARG, "azimuth time UFR: the low-pass window or the crop leaves the arrays");
+    if (rc) {
+        char buf[400];
+        snprintf(buf, sizeof buf,
+                 "azimuth time UFR: the low-pass window or the crop leaves the arrays (tiles %d, shape %d, "
+                 "window [%d, %d), crop [%d, %d) of %d; focused time %.9g s, n_ref %.9g s, mean FM rate %.9g, "
+                 "f_dc %.9g, FM rate at the reference point %.9g)",
+                 out->num_tiles, out->shape, out->lpf_start, out->lpf_end, out->crop_start, out->crop_end,
+                 out->downsample_shape, out->focused_time, out->n_ref, out->ka_mean, dc_mid, fm_ref);
+        return ctx_error(c, S1G_EARG, buf);
+    }
     return S1G_OK;
 }
 

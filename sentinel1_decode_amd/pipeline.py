@@ -335,13 +335,7 @@ class Level0Product:
         rows = u.shape[0]
         t0 = self.packet_time(first)
         dt = (T / prf) / (rows / L)
-        pos = np.zeros((rows, 3))
-        vn = np.zeros(rows)
-        for i in range(rows):
-            st = capi.interpolate_state(self.state_vectors, t0 + dt * i)
-            pos[i] = list(st.position)
-            v = st.velocity
-            vn[i] = np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+        pos, vn = capi.line_states(self.state_vectors, t0, dt, rows)
         geo = geometry_from_info(info, L)
         nr = abs(rate * T / prf)
         dev = self.d_stream.device

@@ -413,7 +413,7 @@ class Level0:
     n_packets: int
 
 
-def level0_stream(specs, t0=1406987562, first_pri=243384, lead_packets=0):
+def level0_stream(specs, t0=1406987562, first_pri=243384, lead_packets=0, beam_addresses=(100, 900)):
     """Concatenate bursts (BurstSpec, encoded in turn) into one Level-0 stream
     in acquisition order.  Each packet's headers are rewritten as the
     instrument writes them: pri_count and space_packet_count count up over
@@ -422,7 +422,9 @@ def level0_stream(specs, t0=1406987562, first_pri=243384, lead_packets=0):
     one, coarse/fine time follow the PRI, and sc_data_word_index cycles
     1..64 carrying the orbit's state vector (one per second of stream).
     `lead_packets` > 0 puts that many packets of the first spec's swath
-    before everything (a partial burst the reader must drop)."""
+    before everything (a partial burst the reader must drop).
+    `beam_addresses` are the first and last azimuth beam addresses of a
+    burst's ramp (IW sweeps about 213..810, +-0.0105 rad)."""
     chunks, bursts = [], []
     pkt = 0
     clock = []  # each packet's time since the stream start (s), one PRI apart
@@ -446,7 +448,8 @@ def level0_stream(specs, t0=1406987562, first_pri=243384, lead_packets=0):
             words = _subcomm_cycle(t0 + int(clock[pkt + l - cycle_pos]))
             s_["pri_count"] = first_pri + pkt + l
             s_["space_packet_count"] = 240533 + pkt + l
-            s_["azimuth_beam_address"] = 100 + (l * 800) // max(n - 1, 1)
+            a0, a1 = beam_addresses
+            s_["azimuth_beam_address"] = a0 + (l * (a1 - a0)) // max(n - 1, 1)
             s_["coarse_time"] = t0 + int(t)
             s_["fine_time"] = max(1, int((t - int(t)) * 65536))
             s_["sc_data_word_index"] = sc_idx
