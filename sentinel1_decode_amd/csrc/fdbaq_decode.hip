@@ -21,7 +21,11 @@
 //   phase 2 lanes take (channel, block) pairs and decode their symbols
 //           serially from the recorded starts, reconstruct the samples in
 //           FP64 (NRL*SF rounded once) and store them interleaved:
-//           out[2k] = IE_k + i QE_k, out[2k+1] = IO_k + i QO_k.
+//           out[2k] = IE_k + i QE_k, out[2k+1] = IO_k + i QO_k.  Lanes
+//           4j..4j+3 take the four channels of one block, so each store
+//           instruction writes whole samples, and a block's values are
+//           decoded together once its QO start is known, following the QO
+//           walk.
 #include "s1g_device.h"
 #include "s1g_fdbaq.h"
 #include "s1g_internal.h"
@@ -224,116 +228,225 @@ __device__ __forceinline__ int select_bit(uint32_t w, int j) {
     return pos;
 }
 
+// One lane's part of a region walk: the region is 64 slices of `delta`
+// bits from `rstart`; lane l owns [rstart + l*delta, rstart + (l+1)*delta).
+struct Region {
+    Win5 w;
+    int zero;       // window offset of slice offset 0
+    uint32_t mask;  // slice offsets where the settled chain's symbols start
+    int wex;        // exit of the lane's recorded walk
+    int ex;         // exit handed to the right neighbour
+    int entry;      // entry of the settled chain (>= delta: none)
+};
+
+// 1. Speculative decode: each lane decodes from kOverlap bits before its
+//    slice (or from rstart if that is nearer), recording in `mask` the slice
+//    offsets where its symbols start, until it leaves the slice.
+template <int BRC, bool ALL>
+__device__ __forceinline__ void region_walk(const PacketWords<ALL>& words, int rstart, int delta, int lane,
+                                            Region& r) {
+    constexpr bool B4 = BRC == 4;
+    const int s = rstart + lane * delta;
+    // BRC 4 codes resynchronise more slowly: a longer lead-in, within the
+    // 64-bit window (lead-in + delta + 11 <= 64)
+    const int lead = B4 ? min(kOverlapB4, 53 - delta) : kOverlap;
+    const int org = max(s - lead, rstart);
+    const int w0 = org >> 5;
+    r.w.w0 = words(w0);
+    r.w.w1 = words(w0 + 1);
+    r.w.w2 = words(w0 + 2);
+    r.w.w3 = words(w0 + 3);
+    r.w.w4 = words(w0 + 4);
+    // window offset of slice offset o is o + zero (zero <= 31 + kOverlap)
+    r.zero = (int)(org & 31) + (int)(s - org);
+    uint64_t win = walk_window(win64(r.w, (int)(org & 31)));
+    uint32_t mask = 0;
+    int o = (int)(org - s);
+    // lead-in: the overlap before the slice (nothing recorded), then the
+    // slice; two symbols per trip, the second under a select rather than a
+    // branch, which halves the loop's exec-mask bookkeeping
+    while (o < 0) {
+        const int l = sym_len_n<BRC>((uint32_t)(win >> 32));
+        o += l;
+        win <<= l;
+        const bool more = o < 0;
+        const int l2 = sym_len_n<BRC>((uint32_t)(win >> 32));
+        o = more ? o + l2 : o;
+        win = more ? win << l2 : win;
+    }
+    while (o < delta) {
+        mask |= 1u << o;
+        const int l = sym_len_n<BRC>((uint32_t)(win >> 32));
+        o += l;
+        win <<= l;
+        const bool more = o < delta;
+        mask |= more ? 1u << (o & 31) : 0u;
+        const int l2 = sym_len_n<BRC>((uint32_t)(win >> 32));
+        o = more ? o + l2 : o;
+        win = more ? win << l2 : win;
+    }
+    r.mask = mask;
+    r.wex = o;
+    r.ex = o;
+    r.entry = 0;
+}
+
+// 2. Settle the chain that enters anchor lane `k` at slice offset `e` (a true
+//    symbol boundary): a lane's entry is its left neighbour's exit; if that
+//    offset is one it recorded, its exit stands, otherwise it walks again from
+//    the entry (a slice that a symbol from the left covers entirely passes the
+//    entry on as its exit); repeat until no exit changes.  Lanes below k take
+//    no part.  Exits settle left to right, at least one more lane per round:
+//    64 rounds suffice, the bound only guards against a malformed table.
+template <int BRC>
+__device__ __forceinline__ void region_settle(Region& r, int k, int e, int delta, int lane) {
+    for (int round = 0; round <= 64; ++round) {
+        const int up = from_left(r.ex);
+        const int entry = lane < k ? 2 * delta : (lane == k ? e : up - delta);
+        bool changed = false;
+        int nex;
+        if (entry >= delta) {
+            nex = entry;  // a symbol from the left covers this whole slice
+        } else {
+            if (!((r.mask >> entry) & 1u)) {
+                uint32_t mask = 0;
+                int o = entry;
+                uint64_t wr = walk_window(win64(r.w, r.zero + entry));
+                while (o < delta) {
+                    mask |= 1u << o;
+                    const int l = sym_len_n<BRC>((uint32_t)(wr >> 32));
+                    o += l;
+                    wr <<= l;
+                    const bool more = o < delta;
+                    mask |= more ? 1u << (o & 31) : 0u;
+                    const int l2 = sym_len_n<BRC>((uint32_t)(wr >> 32));
+                    o = more ? o + l2 : o;
+                    wr = more ? wr << l2 : wr;
+                }
+                r.mask = mask;
+                r.wex = o;
+                changed = true;
+            }
+            nex = r.wex;
+        }
+        changed |= nex != r.ex;
+        r.ex = nex;
+        r.entry = entry;
+        if (__ballot(changed) == 0) break;
+    }
+}
+
+// 3. Count: a prefix sum of per-lane symbol counts of the settled chain finds
+//    the lane holding the nsym-th symbol; the block ends where its next
+//    recorded symbol starts, or at its exit.  Returns false when the region
+//    holds fewer than nsym symbols (*counted of them, the chain leaving the
+//    region at *rend).  With want_hdr (IE, QE) *hdr receives the 8 bits that
+//    follow the end (the next block's BRC or THIDX field).
+__device__ __forceinline__ bool region_count(const Region& r, int rstart, int delta, int nsym, int lane,
+                                             bool want_hdr, int* end, uint32_t* hdr, int* counted, int* rend) {
+    const uint32_t live = r.entry >= delta ? 0u : (r.mask >> r.entry);
+    const int cnt = __popc(live);
+    const int P = wave_inclusive_sum(cnt, lane);
+    const int total = __builtin_amdgcn_readlane(P, 63);
+    if (total >= nsym) {
+        const unsigned long long bal = __ballot(P >= nsym);
+        const int ls = __ffsll((long long)bal) - 1;
+        // every lane forms its candidate without branching; lane ls's is the
+        // block's: k block symbols start in its slice (1..cnt), so the block
+        // ends at its (k+1)-th start, or at its exit when k == cnt
+        const int kk = nsym - (P - cnt);
+        const int endrel = (kk >= 1 && kk < cnt) ? r.entry + select_bit(live, kk + 1) : r.ex;
+        if (want_hdr) {  // uniform: IE and QE blocks are followed by a header field
+            const uint32_t h = (uint32_t)(win64(r.w, min(r.zero + endrel, 95)) >> 56);
+            *hdr = (uint32_t)__builtin_amdgcn_readlane((int)h, ls);
+        } else {
+            *hdr = 0;
+        }
+        *end = rstart + ls * delta + __builtin_amdgcn_readlane(endrel, ls);
+        return true;
+    }
+    *counted = total;
+    *rend = rstart + 63 * delta + __builtin_amdgcn_readlane(r.ex, 63);
+    return false;
+}
+
 // Position after `nsym` symbols of one BRC starting at bit `start`, found by
-// all 64 lanes.  Lane l owns the slice [start + l*delta, start + (l+1)*delta)
-// of a region of 64*delta bits:
-//   1. it decodes speculatively from kOverlap bits before its slice (or from
-//      `start`, a true boundary, if that is nearer), recording in `mask` the
-//      slice offsets where its symbols start, until it leaves the slice;
-//   2. its true entry is its left neighbour's exit; if that offset is one it
-//      recorded, its exit stands, otherwise it walks again from the entry
-//      (a slice that a symbol from the left covers entirely passes the entry
-//      on as its exit); repeat until no exit changes;
-//   3. a prefix sum of per-lane symbol counts finds the lane holding the
-//      nsym-th symbol; the block ends where its next recorded symbol starts,
-//      or at its exit.
-// Regions repeat when one does not hold all nsym symbols.  delta <= 32.
-// With want_hdr (IE, QE) *hdr receives the 8 bits that follow the end (the
-// next block's BRC or THIDX field), else 0.  Reads stay below word `wlim`; once `start` passes `nbits`
-// the block cannot end inside the packet and the walk stops there.
+// all 64 lanes: region walks from `start` (lane 0 enters at offset 0), repeated
+// when one region does not hold all nsym symbols.  delta <= 32.  Reads stay
+// below word `wlim`; once `start` passes `nbits` the block cannot end inside
+// the packet and the walk stops there.
 template <int BRC, bool ALL>
 __device__ int walk_block(const PacketWords<ALL>& words, int nbits, int start, int nsym, int delta, int lane,
                           bool want_hdr, uint32_t* hdr) {
-    constexpr bool B4 = BRC == 4;
     for (;;) {
         if (start > nbits) {
             *hdr = 0;
             return start;
         }
-        const int s = start + lane * delta;
-        // BRC 4 codes resynchronise more slowly: a longer lead-in, within the
-        // 64-bit window (lead-in + delta + 11 <= 64)
-        const int lead = B4 ? min(kOverlapB4, 53 - delta) : kOverlap;
-        const int org = max(s - lead, start);
-        const int w0 = org >> 5;
-        Win5 w;
-        w.w0 = words(w0);
-        w.w1 = words(w0 + 1);
-        w.w2 = words(w0 + 2);
-        w.w3 = words(w0 + 3);
-        w.w4 = words(w0 + 4);
-        // window offset of slice offset o is o + zero (zero <= 31 + kOverlap)
-        const int zero = (int)(org & 31) + (int)(s - org);
-        uint64_t win = walk_window(win64(w, (int)(org & 31)));
-        uint32_t mask = 0;
-        int o = (int)(org - s);
-        // lead-in: the overlap before the slice (nothing recorded)
-        while (o < 0) {
-            const int l = sym_len_n<BRC>((uint32_t)(win >> 32));
-            o += l;
-            win <<= l;
-        }
-        while (o < delta) {
-            mask |= 1u << o;
-            const int l = sym_len_n<BRC>((uint32_t)(win >> 32));
-            o += l;
-            win <<= l;
-        }
-        int wex = o;  // exit of the recorded walk
-        int ex = o;   // exit handed to the right neighbour
-        int entry = 0;
-        // Exits settle left to right, at least one more lane per round: 64
-        // rounds suffice, the bound only guards against a malformed table.
-        for (int round = 0; round <= 64; ++round) {
-            const int up = from_left(ex);
-            entry = lane == 0 ? 0 : up - delta;
-            bool changed = false;
-            int nex;
-            if (entry >= delta) {
-                nex = entry;  // a symbol from the left covers this whole slice
-            } else {
-                if (!((mask >> entry) & 1u)) {
-                    mask = 0;
-                    o = entry;
-                    uint64_t wr = walk_window(win64(w, zero + entry));
-                    while (o < delta) {
-                        mask |= 1u << o;
-                        const int l = sym_len_n<BRC>((uint32_t)(wr >> 32));
-                        o += l;
-                        wr <<= l;
-                    }
-                    wex = o;
-                    changed = true;
-                }
-                nex = wex;
-            }
-            changed |= nex != ex;
-            ex = nex;
-            if (__ballot(changed) == 0) break;
-        }
-        const uint32_t live = entry >= delta ? 0u : (mask >> entry);
-        const int cnt = __popc(live);
-        const int P = wave_inclusive_sum(cnt, lane);
-        const int total = __builtin_amdgcn_readlane(P, 63);
+        Region r;
+        region_walk<BRC, ALL>(words, start, delta, lane, r);
+        region_settle<BRC>(r, 0, 0, delta, lane);
+        int end, counted, rend;
+        if (region_count(r, start, delta, nsym, lane, want_hdr, &end, hdr, &counted, &rend)) return end;
+        nsym -= counted;
+        start = rend;
+    }
+}
+
+// Inclusive prefix over the lanes of the settled chain's symbol counts.
+__device__ __forceinline__ int region_prefix(const Region& r, int delta, int lane) {
+    const uint32_t live = r.entry >= delta ? 0u : (r.mask >> r.entry);
+    return wave_inclusive_sum(__popc(live), lane);
+}
+
+// A block walked from a region speculated before its start was known: the
+// region from `pred` was walked and settled from lane 0 (prefix counts
+// `pre`).  Given the true start `start`:
+//   - if the settled chain passes through it (lane k holds it as a recorded
+//     symbol start at or after its entry), the chain from `start` is that
+//     chain from there on, so its counts are the settled ones shifted by a
+//     constant: the block's end follows from one ballot, without settling
+//     or summing again;
+//   - otherwise the chain is settled again from lane k and counted;
+//   - a start outside the region falls back to a plain walk, and a block
+//     that outruns the region goes on from the region's end.
+template <int BRC, bool ALL>
+__device__ int resolve_block(const PacketWords<ALL>& words, int nbits, Region& r, int pre, int pred, int start,
+                             int nsym, int delta, int lane, bool want_hdr, uint32_t* hdr) {
+    const int off = start - pred;
+    if (start > nbits || off < 0 || off >= 64 * delta)
+        return walk_block<BRC, ALL>(words, nbits, start, nsym, delta, lane, want_hdr, hdr);
+    const int k = off / delta, e = off - k * delta;
+    const uint32_t mk = (uint32_t)__builtin_amdgcn_readlane((int)r.mask, k);
+    const int ek = __builtin_amdgcn_readlane(r.entry, k);
+    if (ek <= e && ((mk >> e) & 1u)) {
+        const int ck = __popc(mk >> e);
+        const int c0 = __builtin_amdgcn_readlane(pre, k) - ck;  // counts from `start`: pre - c0 (lanes >= k)
+        const int total = __builtin_amdgcn_readlane(pre, 63) - c0;
         if (total >= nsym) {
-            const unsigned long long bal = __ballot(P >= nsym);
+            const int P = pre - c0;
+            const unsigned long long bal = __ballot(lane >= k && P >= nsym);
             const int ls = __ffsll((long long)bal) - 1;
-            // every lane forms its candidate without branching; lane ls's is
-            // the block's: k block symbols start in its slice (1..cnt), so the
-            // block ends at its (k+1)-th start, or at its exit when k == cnt
-            const int k = nsym - (P - cnt);
-            const int endrel = (k >= 1 && k < cnt) ? entry + select_bit(live, k + 1) : ex;
-            if (want_hdr) {  // uniform: IE and QE blocks are followed by a header field
-                const uint32_t h = (uint32_t)(win64(w, min(zero + endrel, 95)) >> 56);
+            const int ent = lane == k ? e : r.entry;
+            const uint32_t live = ent >= delta ? 0u : (r.mask >> ent);
+            const int cnt = __popc(live);
+            const int kk = nsym - (P - cnt);
+            const int endrel = (kk >= 1 && kk < cnt) ? ent + select_bit(live, kk + 1) : r.ex;
+            if (want_hdr) {
+                const uint32_t h = (uint32_t)(win64(r.w, min(r.zero + endrel, 95)) >> 56);
                 *hdr = (uint32_t)__builtin_amdgcn_readlane((int)h, ls);
             } else {
                 *hdr = 0;
             }
-            return start + ls * delta + __builtin_amdgcn_readlane(endrel, ls);
+            return pred + ls * delta + __builtin_amdgcn_readlane(endrel, ls);
         }
-        nsym -= total;
-        start += 63 * delta + __builtin_amdgcn_readlane(ex, 63);
+        return walk_block<BRC, ALL>(words, nbits, pred + 63 * delta + __builtin_amdgcn_readlane(r.ex, 63),
+                                    nsym - total, delta, lane, want_hdr, hdr);
     }
+    region_settle<BRC>(r, k, e, delta, lane);
+    int end, counted, rend;
+    if (region_count(r, pred, delta, nsym, lane, want_hdr, &end, hdr, &counted, &rend)) return end;
+    return walk_block<BRC, ALL>(words, nbits, rend, nsym - counted, delta, lane, want_hdr, hdr);
 }
 
 // BRC of block b (uniform) from the lane-distributed registers.
@@ -404,28 +517,75 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
     }
 }
 
-// Two wavefronts per packet.  Wave 0 walks the block structure of IE, IO
-// and QE; QE carries the THIDX every channel's reconstruction needs, so at
-// that barrier wave 1 starts decoding the three channels' values while wave
-// 0 walks QO; after the second barrier both waves decode QO and the rest of
-// the three channels' items.  The walk is the packet's critical path, so the
-// walking wave raises its issue priority (s_setprio) over the value decodes
-// sharing its SIMD.  Only workgroup barriers hand work over, and every wave
-// passes each of them once.
+// Bits the speculative region of a block starts before the predicted start
+// (the previous block's start plus IE's length of that block): channels of
+// one block share its BRC and statistics, so the lengths differ by ~16 bits.
+constexpr int kSpecMargin = 40;
+// Progress value of a finished (or abandoned) channel.
+constexpr int kDone = 1 << 20;
+
+__device__ __forceinline__ int lds_load_acq(int* p) {
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+// Progress only grows (a posted kDone is never taken back).
+__device__ __forceinline__ void lds_store_rel(int* p, int v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits (wave-uniform) until *flag >= need or an error is posted; false on
+// error.  Every wave that others wait on posts its progress, and kDone
+// with the error when it stops, so the wait ends; the bound only guards
+// against a broken invariant (it posts S1G_PKT_ERR_ARG).
+// SLEEP: s_sleep units (64 cycles) between polls -- long for the value
+// wave's wait on the walks, so that its polling takes few issue slots from
+// the walking waves sharing the CU.
+template <int SLEEP>
+__device__ __forceinline__ bool wait_for(int* flag, int need, int* err, int lane) {
+    for (int spin = 0; spin < (1 << 24); ++spin) {
+        if (lds_load_acq(err) != S1G_PKT_OK) return false;
+        if (lds_load_acq(flag) >= need) return true;
+        __builtin_amdgcn_s_sleep(SLEEP);
+    }
+    if (lane == 0) __hip_atomic_store(err, S1G_PKT_ERR_ARG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return false;
+}
+
+// Three wavefronts per packet.
+//   wave 0 walks IE block by block (each block's BRC is in the stream);
+//   waves 0 and 1 then walk IO, QE and QO as a pipeline: block b is walked
+//     by wave b & 1.  Its BRC is known from IE, so before its start is known
+//     the wave speculates: it walks and settles a region from the previous
+//     block's start plus IE's length of that block, less kSpecMargin bits.
+//     When the other wave publishes the start (an LDS word with release /
+//     acquire at workgroup scope), the chain is settled again from the lane
+//     that holds it and counted -- usually one round -- so each block costs
+//     about half a walk on the critical path;
+//   wave 2 decodes values, in groups of 16 blocks (lane 4j + ch takes
+//     channel ch of the group's block j, so each store instruction writes
+//     whole samples), as soon as the group's QO starts are published; waves
+//     0 and 1 join it when the walks end.
+// Errors are found in block order (a block is resolved only after the one
+// before it), posted once, and end every wait.  The walks are the packet's
+// critical path, so the walking waves raise their issue priority
+// (s_setprio) over the value decodes sharing their SIMDs.
 template <typename T, bool ALL>
-__global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
+__global__ __launch_bounds__(192) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
                                                            const uint64_t* __restrict__ offs,
                                                            const uint32_t* __restrict__ lens,
                                                            const int32_t* __restrict__ nqs, int n_packets,
                                                            int max_words, int lds_words, int max_blocks,
                                                            int max_quads, T* __restrict__ out, int64_t ld_out,
                                                            int32_t* __restrict__ status) {
+    constexpr int kThreads = 192;
     extern __shared__ __align__(16) uint32_t smem[];  // base 16-aligned after the static words
-    __shared__ int s_st, s_st_qo;  // status after IE..QE; status of the QO walk
+    __shared__ int s_err;          // first error status
+    __shared__ int s_pub[4];       // blocks of each channel whose start is published; kDone when finished
+    __shared__ int s_chan_end[4];  // end of each channel's last block, before the 16-bit alignment
+    __shared__ int s_next_group;   // next value-decode group
     const int p = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63;
-    // wave index as a scalar: the block walk under `wave == 0` is then
-    // uniform control flow rather than exec-masked
+    // wave index as a scalar: the walks under `wave == ...` are then uniform
+    // control flow rather than exec-masked
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (p >= n_packets) return;
 
@@ -438,7 +598,7 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
     // which wait on every outstanding global store)
     const int nrl_words = (lds_words + 4 * max_blocks + (2 * max_blocks + 3) / 4 + 1) & ~1;
     double* nrl = (double*)(smem + nrl_words);
-    for (int i = tid; i < 80; i += 128) nrl[i] = c_nrl[i >> 4][i & 15];
+    for (int i = tid; i < 80; i += kThreads) nrl[i] = c_nrl[i >> 4][i & 15];
 
     const int nq = nqs[p];
     const uint32_t nbytes_u = lens[p];
@@ -473,127 +633,216 @@ __global__ __launch_bounds__(128) void fdbaq_decode_kernel(const uint8_t* __rest
         const int n_stage = min(lds_words, words.wlim);
         const __amdgpu_buffer_rsrc_t prs = buf_rsrc(words.al, 4u * (uint32_t)(words.last_al + 1));
         constexpr int kStageBatch = 8;
-        for (int b = 0; b < n_stage; b += 128 * kStageBatch) {
+        for (int b = 0; b < n_stage; b += kThreads * kStageBatch) {
             uint32_t lo[kStageBatch], hi[kStageBatch];
 #pragma unroll
             for (int u = 0; u < kStageBatch; ++u) {
-                const int i = b + tid + 128 * u;
+                const int i = b + tid + kThreads * u;
                 const uint32_t o = i < n_stage ? 4u * (uint32_t)i : kBufSkip;
                 lo[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(prs, (int)o, 0, 0);
                 hi[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(prs, (int)(o + 4u), 0, 0);
             }
 #pragma unroll
             for (int u = 0; u < kStageBatch; ++u) {
-                const int i = b + tid + 128 * u;
+                const int i = b + tid + kThreads * u;
                 if (i < n_stage) staged[i] = words.word_of(lo[u], hi[u], i);
             }
         }
     }
     if (tid == 0) {
-        s_st = st;
-        s_st_qo = S1G_PKT_OK;
+        s_err = st;
+        for (int c = 0; c < 4; ++c) {
+            s_pub[c] = 0;
+            s_chan_end[c] = 0;
+        }
+        s_next_group = 0;
     }
     __syncthreads();
 
-    // ---- phase 1 (wave 0): block structure, uniform control flow
     const int nbits = 8 * nbytes;
-    int pos = 0;
-    int prev_len = 0;
-    uint32_t vb = 0;  // bits 3j..3j+2: BRC of block lane + 64 j
-    auto walk_channel = [&](int ch) __attribute__((always_inline)) {
-        // header field of the channel's first block from the packet; later
-        // ones come out of the walk of the block before
-        const bool hdr_field = ch == 0 || ch == 2;
-        uint32_t hdr = hdr_field ? peek_bits(words, pos, 8) : 0u;
-        for (int b = 0; b < nb && st == S1G_PKT_OK; ++b) {
-            int brc;
-            if (ch == 0) {
-                if (pos + 3 > nbits) {
-                    st = S1G_PKT_ERR_OVERRUN;
-                    break;
+    // posts error `e` (first one wins) and ends every wait
+    auto fail = [&](int e) __attribute__((always_inline)) {
+        if (lane == 0) {
+            int expect = S1G_PKT_OK;
+            __hip_atomic_compare_exchange_strong(&s_err, &expect, e, __ATOMIC_RELEASE, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            for (int c = 0; c < 4; ++c) lds_store_rel(&s_pub[c], kDone);
+        }
+    };
+    // the block walk of one BRC (uniform switch)
+    auto walk_any = [&](int brc, int start, int nsym, int delta, bool want_hdr, uint32_t* hdr)
+                        __attribute__((always_inline)) {
+        switch (brc) {
+            case 0: return walk_block<0, ALL>(words, nbits, start, nsym, delta, lane, want_hdr, hdr);
+            case 1: return walk_block<1, ALL>(words, nbits, start, nsym, delta, lane, want_hdr, hdr);
+            case 2: return walk_block<2, ALL>(words, nbits, start, nsym, delta, lane, want_hdr, hdr);
+            case 3: return walk_block<3, ALL>(words, nbits, start, nsym, delta, lane, want_hdr, hdr);
+            default: return walk_block<4, ALL>(words, nbits, start, nsym, delta, lane, want_hdr, hdr);
+        }
+    };
+
+    if (s_err == S1G_PKT_OK && wave == 0) {
+        // ---- IE, block by block: the BRC of each block heads it
+        __builtin_amdgcn_s_setprio(2);
+        int pos = 0, prev_len = 0;
+        uint32_t hdr = peek_bits(words, pos, 8);
+        bool ok = true;
+        for (int b = 0; b < nb; ++b) {
+            if (pos + 3 > nbits) {
+                fail(S1G_PKT_ERR_OVERRUN);
+                ok = false;
+                break;
+            }
+            const uint32_t v = hdr >> 5;
+            if (v > 4) {
+                fail(S1G_PKT_ERR_BRC);
+                ok = false;
+                break;
+            }
+            if (lane == 0) bbrc[b] = (uint8_t)v;
+            pos += 3;
+            if (lane == 0) bstart[b] = (int32_t)pos;
+            const int nsym = (b == nb - 1) ? nq - 128 * (nb - 1) : 128;
+            // slice width from the previous block (or a BRC guess)
+            const int est = b ? prev_len : nsym * ((int)v + 3);
+            const int delta = min(max((est + 63) / 64 + 1, 2), 32);
+            const int end = walk_any((int)v, pos, nsym, delta, true, &hdr);
+            prev_len = end - pos;
+            pos = end;
+            if (pos > nbits) {
+                fail(S1G_PKT_ERR_OVERRUN);
+                ok = false;
+                break;
+            }
+        }
+        if (ok && lane == 0) {
+            s_chan_end[0] = pos;
+            lds_store_rel(&s_pub[0], kDone);
+        }
+        __builtin_amdgcn_s_setprio(0);
+    }
+
+    if (wave < 2) {
+        // ---- IO, QE, QO: blocks alternate between waves 0 and 1
+        __builtin_amdgcn_s_setprio(2);
+        for (int ch = 1; ch < 4; ++ch) {
+            if (!wait_for<4>(&s_pub[ch - 1], kDone, &s_err, lane)) break;
+            if (lds_load_acq(&s_err) != S1G_PKT_OK) break;
+            // IE's length of block b, the estimate for every channel's block b
+            auto ie_len = [&](int b) __attribute__((always_inline)) {
+                const int e = b + 1 < nb ? bstart[b + 1] - 3 : s_chan_end[0];
+                return __builtin_amdgcn_readfirstlane(e - bstart[b]);
+            };
+            int s0 = (__builtin_amdgcn_readfirstlane(s_chan_end[ch - 1]) + 15) & ~15;
+            const int hbits = ch == 2 ? 8 : 0;
+            if (nb == 0) {  // no blocks: the channel is empty
+                if (wave == 0 && lane == 0) {
+                    s_chan_end[ch] = s0;
+                    lds_store_rel(&s_pub[ch], kDone);
                 }
-                const uint32_t v = hdr >> 5;
-                if (v > 4) {
-                    st = S1G_PKT_ERR_BRC;
-                    break;
-                }
-                brc = (int)v;
-                if (lane == (b & 63)) vb |= v << (3 * (b >> 6));
-                if (lane == 0) bbrc[b] = (uint8_t)v;
-                pos += 3;
-            } else {
-                brc = brc_of(b, vb);
+                continue;
             }
             if (ch == 2) {
-                if (pos + 8 > nbits) {
-                    st = S1G_PKT_ERR_OVERRUN;
+                if (s0 + 8 > nbits) {
+                    if (wave == 0) fail(S1G_PKT_ERR_OVERRUN);
                     break;
                 }
-                if (lane == 0) bthr[b] = (uint8_t)hdr;
-                pos += 8;
+                if (wave == 0 && lane == 0) bthr[0] = (uint8_t)peek_bits(words, s0, 8);
+                s0 += 8;
             }
-            if (lane == 0) bstart[ch * max_blocks + b] = (int32_t)pos;
-            const int nsym = (b == nb - 1) ? nq - 128 * (nb - 1) : 128;
-            // slice width from the previous block of this channel (or a BRC guess)
-            const int est = b ? prev_len : nsym * (brc + 3);
-            const int delta = min(max((est + 63) / 64 + 1, 2), 32);
-            int end;
-            switch (brc) {
-                case 0: end = walk_block<0, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
-                case 1: end = walk_block<1, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
-                case 2: end = walk_block<2, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
-                case 3: end = walk_block<3, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
-                default: end = walk_block<4, ALL>(words, nbits, pos, nsym, delta, lane, hdr_field, &hdr); break;
+            if (wave == 0 && lane == 0) {
+                bstart[ch * max_blocks] = s0;
+                lds_store_rel(&s_pub[ch], 1);
             }
-            prev_len = (int)(end - pos);
-            pos = end;
-            if (pos > nbits) st = S1G_PKT_ERR_OVERRUN;
+            int base = s0;  // start of the block before this wave's next one
+            bool stop = false;
+            for (int b = wave; b < nb && !stop; b += 2) {
+                const int brc = __builtin_amdgcn_readfirstlane((int)bbrc[b]);
+                const int nsym = (b == nb - 1) ? nq - 128 * (nb - 1) : 128;
+                uint32_t hdr = 0;
+                int end;
+                if (b == 0) {
+                    const int delta = min(max((ie_len(0) + 63) / 64 + 1, 2), 32);
+                    end = walk_any(brc, s0, nsym, delta, ch == 2, &hdr);
+                } else {
+                    // clamped to the data: a region from at most nbits reads
+                    // no further than the staged zero words reach
+                    const int pred = min(max(base + ie_len(b - 1) + hbits - kSpecMargin, base), nbits);
+                    const int delta = min(max((ie_len(b) + 2 * kSpecMargin + 16 + 63) / 64 + 1, 2), 32);
+                    Region r;
+                    int pre;
+                    switch (brc) {
+                        case 0: region_walk<0, ALL>(words, pred, delta, lane, r); region_settle<0>(r, 0, 0, delta, lane); break;
+                        case 1: region_walk<1, ALL>(words, pred, delta, lane, r); region_settle<1>(r, 0, 0, delta, lane); break;
+                        case 2: region_walk<2, ALL>(words, pred, delta, lane, r); region_settle<2>(r, 0, 0, delta, lane); break;
+                        case 3: region_walk<3, ALL>(words, pred, delta, lane, r); region_settle<3>(r, 0, 0, delta, lane); break;
+                        default: region_walk<4, ALL>(words, pred, delta, lane, r); region_settle<4>(r, 0, 0, delta, lane); break;
+                    }
+                    pre = region_prefix(r, delta, lane);
+                    if (!wait_for<1>(&s_pub[ch], b + 1, &s_err, lane)) break;
+                    const int start = __builtin_amdgcn_readfirstlane(bstart[ch * max_blocks + b]);
+                    switch (brc) {
+                        case 0: end = resolve_block<0, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
+                        case 1: end = resolve_block<1, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
+                        case 2: end = resolve_block<2, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
+                        case 3: end = resolve_block<3, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
+                        default: end = resolve_block<4, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
+                    }
+                }
+                if (end > nbits) {
+                    fail(S1G_PKT_ERR_OVERRUN);
+                    stop = true;
+                    break;
+                }
+                int next = end;
+                if (b + 1 < nb) {
+                    if (ch == 2) {
+                        if (next + 8 > nbits) {
+                            fail(S1G_PKT_ERR_OVERRUN);
+                            stop = true;
+                            break;
+                        }
+                        if (lane == 0) bthr[b + 1] = (uint8_t)hdr;
+                        next += 8;
+                    }
+                    if (lane == 0) {
+                        bstart[ch * max_blocks + b + 1] = next;
+                        lds_store_rel(&s_pub[ch], b + 2);
+                    }
+                } else if (lane == 0) {
+                    s_chan_end[ch] = end;
+                    lds_store_rel(&s_pub[ch], kDone);
+                }
+                base = next;
+            }
+            if (stop) break;
         }
-        pos = (pos + 15) & ~15;
-    };
-    if (wave == 0 && st == S1G_PKT_OK) {
-        __builtin_amdgcn_s_setprio(2);
-        for (int ch = 0; ch < 3 && st == S1G_PKT_OK; ++ch) walk_channel(ch);
         __builtin_amdgcn_s_setprio(0);
-        if (lane == 0 && st != S1G_PKT_OK) s_st = st;
     }
-    __syncthreads();
-    st = s_st;
-    // IE/IO/QE values: wave 1 decodes items [0, n2) beside the QO walk, the
-    // rest fill the lanes the QO blocks leave free in the last round (128
-    // lanes for nb QO blocks), so that round stays one round
-    const int n2 = min(3 * nb, max(4 * nb - 128, 0));
-    if (st == S1G_PKT_OK) {
-        if (wave == 0) {
-            // the QO walk is the packet's critical path; the value decodes
-            // beside it (this packet's and other packets') have slack, so
-            // the walking wave takes issue priority on its SIMD meanwhile
-            __builtin_amdgcn_s_setprio(2);
-            walk_channel(3);
-            __builtin_amdgcn_s_setprio(0);
-            if (lane == 0 && st != S1G_PKT_OK) s_st_qo = st;
-        } else {
-            // ---- phase 2 (wave 1): IE, IO, QE items [0, n2) while wave 0
-            // walks QO
-            for (int item = lane; item < n2; item += 64)
-                decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, item % 3, item / 3, orow);
+
+    // ---- values, by groups of 16 blocks once their QO starts are published
+    {
+        const int n_groups = (nb + 15) / 16;
+        const bool walked_qe = wait_for<16>(&s_pub[2], kDone, &s_err, lane);
+        for (;;) {
+            if (!walked_qe) break;
+            int g = 0;
+            if (lane == 0)
+                g = __hip_atomic_fetch_add(&s_next_group, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            g = __builtin_amdgcn_readfirstlane(g);
+            if (g >= n_groups) break;
+            if (!wait_for<8>(&s_pub[3], min(16 * g + 16, nb), &s_err, lane)) break;
+            const int b = 16 * g + (lane >> 2);
+            if (b < nb) decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, lane & 3, b, orow);
         }
     }
     __syncthreads();
-    st = s_st != S1G_PKT_OK ? s_st : s_st_qo;
+    st = s_err;
     const int N = 2 * (nq > 0 ? nq : 0);
     if (st != S1G_PKT_OK) {
-        for (int i = tid; i < 2 * N && i < 2 * ld_out; i += 128) orow[i] = (T)0;
+        for (int i = tid; i < 2 * N && i < 2 * ld_out; i += kThreads) orow[i] = (T)0;
         if (tid == 0) status[p] = st;
         return;
-    }
-    // ---- phase 3 (both waves): QO, then the IE/IO/QE items from n2 on
-    const int n3 = nb + (3 * nb - n2);
-    for (int t = tid; t < n3; t += 128) {
-        const int item = n2 + (t - nb);
-        const int ch = t < nb ? 3 : item % 3;
-        const int b = t < nb ? t : item / 3;
-        decode_item(words, bstart, bbrc, bthr, nrl, max_blocks, nq, nb, ch, b, orow);
     }
     if (tid == 0) status[p] = S1G_PKT_OK;
 }
@@ -610,7 +859,7 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     const int nrl_words = (lds_words + 4 * max_blocks + (2 * max_blocks + 3) / 4 + 1) & ~1;
     const size_t lds = sizeof(uint32_t) * (size_t)nrl_words + sizeof(double) * 80;
     if (lds > 160 * 1024) return -1;
-    dim3 grid(n_packets), block(128);
+    dim3 grid(n_packets), block(192);
     const bool all = max_words + kPadWords <= kLdsWords;
 #define S1G_LAUNCH(T, A)                                                                                       \
     hipLaunchKernelGGL((fdbaq_decode_kernel<T, A>), grid, block, lds, stream, ud, offs, lens, nqs, n_packets, \
