@@ -438,7 +438,9 @@ int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_
     if (fsz < 4 || fsz > 1024 || (fsz & (fsz - 1)) || d.stride < 1 || d.stride > fsz)
         return fail(c, S1G_EUNSUPPORTED, "ECCM window must be a power of two in [4, 1024] with 1 <= stride <= size%s");
     if ((fsz + d.stride - 1) / d.stride > 8) return fail(c, S1G_EUNSUPPORTED, "ECCM window overlap too deep%s");
-    // detection split r: N/r must fit 128 KB of LDS and factor over the radices
+    // detection split r: N/r must fit 128 KB of LDS and factor over the
+    // radices (a 64 KB bound, r = 3 instead of 2 for the IW2 FP32 line, gave
+    // two workgroups per CU but 0.70 against 0.61 ms: each reads the line)
     int r = 0;
     static const int cand[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
     for (int k : cand) {

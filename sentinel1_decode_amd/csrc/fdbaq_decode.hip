@@ -82,16 +82,21 @@ constexpr int kMaxPacketBytes = 1 << 27;
 // kOverlap + 32 (the widest slice) + 11 (the longest symbol) <= 64 keeps
 // every walk inside one 64-bit window.
 constexpr int kOverlap = 12;
-constexpr int kOverlapB4 = 24;
+constexpr int kOverlapB4 = 40;  // swept on an RFI burst (mostly BRC 4): 24, 32, 40, 48 bits gave 1.82, 1.70, 1.68, 1.75 ms
 
 // Packets with more BAQ blocks than this are rejected (S1G_PKT_ERR_ARG):
 // the block BRCs are kept in 4 registers per lane.
 constexpr int kMaxBlocks = 256;
 
-// Words of LDS a wavefront stages: packets up to this size (20 KB) are
-// staged whole, so at least 7 wavefronts fit a CU whatever the largest
-// packet of the launch; words beyond it come from global memory on demand.
-constexpr int kLdsWords = 5120;
+// Words of LDS a workgroup stages: packets up to this size (64 KB, more than
+// a space packet's user data can hold) are staged whole.  The launch sizes
+// the LDS for its largest packet, so a burst of ~19 KB packets keeps 7
+// workgroups per CU; one with RFI (larger codes, ~25-30 KB packets) keeps
+// fewer but still reads every word from LDS -- words from global memory
+// cost the walk a memory latency per window (2.0 ms against 0.6 ms per
+// burst with a 20 KB cap).  Larger packets take their words beyond the cap
+// from global memory on demand.
+constexpr int kLdsWords = 16384 + 80;
 
 // A packet's user data as big-endian 32-bit words: word i holds bytes
 // 4i..4i+3, zero past the packet (and word `wlim` onwards).  Words below
