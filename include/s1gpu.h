@@ -580,6 +580,16 @@ int s1g_set_autotune(s1g_ctx* ctx, int on);
 int s1g_force_variant(s1g_ctx* ctx, int stage, int variant);
 /* The variant the stage's last call used and how many it had to choose from. */
 int s1g_last_variant(const s1g_ctx* ctx, int stage, int* variant, int* n_variants);
+/* The tuning cache as a file (host functions): save writes every choice of
+ * the context -- (stage, precision, shape) -> variant, the number of
+ * variants it was chosen among and its time -- under a header naming the
+ * device; load merges a file saved on the same kind of device (S1G_EUNSUPPORTED
+ * otherwise) and its entries are used instead of timing, as long as the
+ * stage still offers the same number of variants.  With the environment
+ * variable S1G_TUNE_CACHE set, s1g_create loads that file (when present) and
+ * s1g_destroy saves the context's cache to it. */
+int s1g_tune_save(const s1g_ctx* ctx, const char* path);
+int s1g_tune_load(s1g_ctx* ctx, const char* path);
 
 #ifdef __cplusplus
 }

@@ -151,6 +151,8 @@ _SIGS = [
     ("s1g_azimuth_time_ufr", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int64, C.c_void_p,
                                        C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                        C.c_void_p, C.c_int64]),
+    ("s1g_tune_save", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("s1g_tune_load", C.c_int, [C.c_void_p, C.c_char_p]),
     ("s1g_set_autotune", C.c_int, [C.c_void_p, C.c_int]),
     ("s1g_force_variant", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("s1g_last_variant", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
@@ -502,6 +504,14 @@ class Context:
         """Stage name (see STAGES) or index; variant -1 returns to automatic choice."""
         st = STAGES.index(stage) if isinstance(stage, str) else int(stage)
         self._check(lib().s1g_force_variant(self.h, st, int(variant)), "force_variant")
+
+    def tune_save(self, path):
+        """Write the context's autotuning choices to `path`."""
+        self._check(lib().s1g_tune_save(self.h, os.fsencode(path)), "tune_save")
+
+    def tune_load(self, path):
+        """Merge the choices saved in `path` (same kind of device)."""
+        self._check(lib().s1g_tune_load(self.h, os.fsencode(path)), "tune_load")
 
     def last_variant(self, stage):
         """(variant used by the stage's last call, number of variants it had)."""

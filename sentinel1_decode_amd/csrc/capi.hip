@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <new>
@@ -96,7 +97,12 @@ struct s1g_ctx {
     int force[S1G_NSTAGES] = {-1, -1, -1, -1, -1, -1, -1, -1};
     int last_var[S1G_NSTAGES] = {0};
     int last_nvar[S1G_NSTAGES] = {0};
-    std::map<TuneKey, int> tuned;
+    struct Tuned {
+        int variant, n_variants;
+        float ms;  // the winner's time when it was tuned (0: loaded without one)
+    };
+    std::map<TuneKey, Tuned> tuned;
+    std::string tune_file;  // S1G_TUNE_CACHE at creation: loaded then, saved at destruction
 };
 
 // Brackets one stage's launches with events when profiling is on.
@@ -281,8 +287,8 @@ int choose_variant(s1g_ctx* c, const TuneKey& k, int nvar, F&& run) {
         v = std::min(c->force[k.stage], nvar - 1);
     } else {
         auto it = c->tuned.find(k);
-        if (it != c->tuned.end()) {
-            v = it->second;
+        if (it != c->tuned.end() && it->second.n_variants == nvar) {
+            v = it->second.variant;
         } else if (c->autotune && !c->capturing && nvar > 1) {
             hipEvent_t a = nullptr, b = nullptr;
             if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
@@ -303,7 +309,7 @@ int choose_variant(s1g_ctx* c, const TuneKey& k, int nvar, F&& run) {
                         v = cand;
                     }
                 }
-                c->tuned[k] = v;
+                c->tuned[k] = s1g_ctx::Tuned{v, nvar, best};
             }
             if (a) hipEventDestroy(a);
             if (b) hipEventDestroy(b);
@@ -494,7 +500,7 @@ int ctx_rcmc_variant(s1g_ctx* c, int prec, int n_lines, int n_samples, int apply
         v = c->force[S1G_STAGE_RCMC_MF];
     } else {
         auto it = c->tuned.find(TuneKey{S1G_STAGE_RCMC_MF, prec, n_lines, n_samples, apply_rcmc});
-        if (it != c->tuned.end()) v = it->second;
+        if (it != c->tuned.end() && it->second.n_variants == nvar) v = it->second.variant;
     }
     v = std::max(0, std::min(v, nvar - 1));
     c->last_var[S1G_STAGE_RCMC_MF] = v;
@@ -520,6 +526,11 @@ int s1g_create(s1g_ctx** out, int device) {
         return S1G_EHIP;
     }
     c->stream = c->own;
+    if (const char* path = getenv("S1G_TUNE_CACHE")) {
+        c->tune_file = path;
+        (void)s1g_tune_load(c, path);  // a missing or foreign file leaves the cache empty
+        c->err.clear();
+    }
     *out = c;
     return S1G_OK;
 }
@@ -528,6 +539,7 @@ int s1g_destroy(s1g_ctx* c) {
     if (!c) return S1G_OK;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    if (!c->tune_file.empty()) (void)s1g_tune_save(c, c->tune_file.c_str());
     for (auto& m : c->marks) {
         hipEventDestroy(m.a);
         hipEventDestroy(m.b);
@@ -579,6 +591,66 @@ int s1g_synchronize(s1g_ctx* c) {
 int s1g_profile_enable(s1g_ctx* c, int on) {
     if (!c) return S1G_EARG;
     c->profiling = on != 0;
+    return S1G_OK;
+}
+
+namespace {
+// The tuning cache file: a header naming the device it was measured on,
+// then one line per (stage, precision, shape): the chosen variant, the
+// number of variants it was chosen among, and its time.
+std::string tune_device_tag(int device) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return "unknown";
+    char buf[160];
+    snprintf(buf, sizeof buf, "%s:%d", p.gcnArchName, p.multiProcessorCount);
+    for (char* q = buf; *q; ++q)
+        if (*q == ' ') *q = '_';
+    return buf;
+}
+}  // namespace
+
+int s1g_tune_save(const s1g_ctx* c, const char* path) {
+    if (!c || !path) return S1G_EARG;
+    FILE* f = fopen(path, "w");
+    if (!f) return S1G_EARG;
+    fprintf(f, "s1gpu-tune 1 %s %zu\n", tune_device_tag(c->device).c_str(), c->tuned.size());
+    for (const auto& kv : c->tuned)
+        fprintf(f, "%d %d %d %d %d %d %d %.6f\n", kv.first.stage, kv.first.prec, kv.first.a, kv.first.b, kv.first.c,
+                kv.second.variant, kv.second.n_variants, (double)kv.second.ms);
+    return fclose(f) == 0 ? S1G_OK : S1G_EARG;
+}
+
+int s1g_tune_load(s1g_ctx* c, const char* path) {
+    if (!c || !path) return S1G_EARG;
+    FILE* f = fopen(path, "r");
+    if (!f) return fail(c, S1G_EARG, "cannot open the tuning cache file%s");
+    char tag[160] = {0};
+    int version = 0;
+    size_t n = 0;
+    if (fscanf(f, "s1gpu-tune %d %159s %zu", &version, tag, &n) != 3 || version != 1) {
+        fclose(f);
+        return fail(c, S1G_EARG, "not a tuning cache file%s");
+    }
+    if (tune_device_tag(c->device) != tag) {
+        fclose(f);
+        return fail(c, S1G_EUNSUPPORTED, "tuning cache file measured on another device%s");
+    }
+    std::map<TuneKey, s1g_ctx::Tuned> got;
+    for (size_t i = 0; i < n; ++i) {
+        TuneKey k;
+        s1g_ctx::Tuned t;
+        double ms = 0;
+        if (fscanf(f, "%d %d %d %d %d %d %d %lf", &k.stage, &k.prec, &k.a, &k.b, &k.c, &t.variant, &t.n_variants,
+                   &ms) != 8 ||
+            k.stage < 0 || k.stage >= S1G_NSTAGES || t.variant < 0 || t.variant >= t.n_variants) {
+            fclose(f);
+            return fail(c, S1G_EARG, "malformed tuning cache file%s");
+        }
+        t.ms = (float)ms;
+        got[k] = t;
+    }
+    fclose(f);
+    for (const auto& kv : got) c->tuned[kv.first] = kv.second;
     return S1G_OK;
 }
 

@@ -200,3 +200,38 @@ def test_autotuned_chain_and_forced_graph(vctx):
         vctx.focus_batch([desc], F32, graph=True)
         torch.cuda.synchronize()
         assert torch.equal(out, forced), rep
+
+
+def test_tuning_cache_file(tmp_path):
+    """Choices timed in one context, saved, and loaded into a context that
+    may not time anything, give the same variants and the same bits; a file
+    from another kind of device and a malformed one are refused."""
+    burst = synth.make_burst(synth.config0(n_lines=256, n_samples=2048))
+    db = pipeline.from_synth(burst)
+    a = capi.Context(0)
+    out_a = db.focus(a, F32).clone()
+    torch.cuda.synchronize()
+    stages = ("range", "az_fft", "rcmc_mf", "az_ifft")
+    chosen = {s: a.last_variant(s) for s in stages}
+    path = tmp_path / "tune.txt"
+    a.tune_save(str(path))
+    a.close()
+    text = path.read_text().splitlines()
+    assert text[0].startswith("s1gpu-tune 1 ") and len(text) == 1 + int(text[0].split()[-1])
+    b = capi.Context(0)
+    b.set_autotune(False)
+    b.tune_load(str(path))
+    out_b = db.focus(b, F32)
+    torch.cuda.synchronize()
+    assert {s: b.last_variant(s) for s in stages} == chosen
+    assert torch.equal(out_a, out_b)
+    head = text[0].split()
+    foreign = tmp_path / "foreign.txt"
+    foreign.write_text(" ".join(head[:2] + ["gfx000:1"] + head[3:]) + "\n" + "\n".join(text[1:]) + "\n")
+    with pytest.raises(capi.S1GError):
+        b.tune_load(str(foreign))
+    bad = tmp_path / "bad.txt"
+    bad.write_text(text[0] + "\n1 2 3\n")
+    with pytest.raises(capi.S1GError):
+        b.tune_load(str(bad))
+    b.close()
