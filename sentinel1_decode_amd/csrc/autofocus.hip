@@ -9,7 +9,9 @@
 //   E = -sum (I/S) log(I/S) = log S - T / S.
 // Adjoint (dE/dI = -log(I)/S + T/S^2 =: w; the IFFT/L is adjointed by FFT/L):
 //   g = FFT_az(w * img) / L,  dE/dphi_i = 2 sum_j Im(g_ij conj(rd_ij e^{i phi_i})).
-// Column panels in LDS as in the azimuth transforms; sums in FP64.
+// The spectrum rd is the checkpoint: the image is never stored between the
+// two passes; the gradient pass recomputes it from rd (one read of rd per
+// pass, nothing written but per-workgroup partial sums).  Sums in FP64.
 #include "s1g_internal.h"
 
 namespace s1g {
@@ -147,52 +149,66 @@ __global__ __launch_bounds__(NT) void af_adjoint_kernel(const cpx<T>* __restrict
     }
 }
 
-__global__ void af_reduce_grad_kernel(const double* __restrict__ gpart, int npanels, int L, double* __restrict__ grad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L) return;
-    double s = 0;
-    for (int p = 0; p < npanels; ++p) s += gpart[(int64_t)p * L + i];
-    grad[i] = s;
-}
-
 template <typename T>
 static int run_af(hipStream_t s, int L, int N, const void* rd, int64_t ld, const double* phase, const DevFft& fl,
-                  void* img, double* work, int want_grad, double* grad, double* scal) {
+                  void* img, void* img_tmp, double* work, int want_grad, double* grad, double* scal) {
+    constexpr int prec = sizeof(T) == 4 ? S1G_F32 : S1G_F64;
+    cpx<T>* ph = (cpx<T>*)work;
+    double* part = work + (size_t)L * 2;
+    hipLaunchKernelGGL((af_phase_table_kernel<T>), dim3((L + 255) / 256), dim3(256), 0, s, phase, ph, L);
+    if (af_fast(L, N, ld, prec)) {
+        // work: phase table | 2 G entropy partials | G L gradient partials
+        const int G = af_reg_workgroups(N, prec);
+        double* gpart = part + 2 * (size_t)G;
+        int rc = launch_af_reg(s, prec, 0, L, N, rd, ld, fl, ph, scal, img, part, gpart);
+        hipLaunchKernelGGL(af_reduce_scalars_kernel, dim3(1), dim3(256), 0, s, part, G, scal);
+        if (!rc && want_grad) {
+            rc = launch_af_reg(s, prec, 2, L, N, rd, ld, fl, ph, scal, nullptr, part, gpart);
+            if (!rc) rc = launch_af_sum_partials(s, gpart, G, L, grad);
+        }
+        return rc ? rc : (int)hipGetLastError();
+    }
+    // other lengths: one workgroup per panel through the plan's LDS passes,
+    // the image kept in img_tmp between the two kernels
     constexpr int NT = 512;
     int W = sizeof(T) == 4 ? 8 : 4;
     while (W > 1 && (size_t)L * W * sizeof(cpx<T>) > 150 * 1024) W /= 2;
     if ((size_t)L * W * sizeof(cpx<T>) > 150 * 1024) return -3;
     const int npanels = (N + W - 1) / W;
     const int per = (npanels + 7) / 8;
-    // work layout: phase table (L cpx<T>, padded to doubles) | 2*npanels partials | npanels*L grad partials
-    cpx<T>* ph = (cpx<T>*)work;
-    double* part = work + (size_t)L * 2;
+    // work: phase table | 2 npanels partials | npanels L gradient partials
     double* gpart = part + 2 * (size_t)npanels;
-    hipLaunchKernelGGL((af_phase_table_kernel<T>), dim3((L + 255) / 256), dim3(256), 0, s, phase, ph, L);
     const size_t lds = (size_t)L * W * sizeof(cpx<T>);
     hipLaunchKernelGGL((af_forward_kernel<T, NT>), dim3(8 * per), dim3(NT), lds, s, (const cpx<T>*)rd, ld,
-                       (cpx<T>*)img, L, N, W, npanels, fl.plan, tw_of<T>(fl), fl.perm, ph, part);
+                       (cpx<T>*)img_tmp, L, N, W, npanels, fl.plan, tw_of<T>(fl), fl.perm, ph, part);
     hipLaunchKernelGGL(af_reduce_scalars_kernel, dim3(1), dim3(256), 0, s, part, npanels, scal);
     if (want_grad) {
         hipLaunchKernelGGL((af_adjoint_kernel<T, NT>), dim3(8 * per), dim3(NT), lds, s, (const cpx<T>*)rd, ld,
-                           (const cpx<T>*)img, L, N, W, npanels, fl.plan, tw_of<T>(fl), fl.perm, ph, scal, gpart);
-        hipLaunchKernelGGL(af_reduce_grad_kernel, dim3((L + 255) / 256), dim3(256), 0, s, gpart, npanels, L, grad);
+                           (const cpx<T>*)img_tmp, L, N, W, npanels, fl.plan, tw_of<T>(fl), fl.perm, ph, scal,
+                           gpart);
+        const int rc = launch_af_sum_partials(s, gpart, npanels, L, grad);
+        if (rc) return rc;
     }
     return (int)hipGetLastError();
 }
 
-size_t af_work_doubles(int L, int N, int prec) {
-    int W = prec == S1G_F32 ? 8 : 4;
-    const size_t esz = prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
-    while (W > 1 && (size_t)L * W * esz > 150 * 1024) W /= 2;
-    const size_t npanels = (size_t)(N + W - 1) / W;
-    return (size_t)L * 2 + 2 * npanels + npanels * (size_t)L + 8;
+size_t af_work_doubles(int L, int N, int64_t ld, int prec) {
+    size_t nparts;
+    if (af_fast(L, N, ld, prec)) {
+        nparts = (size_t)af_reg_workgroups(N, prec);
+    } else {
+        int W = prec == S1G_F32 ? 8 : 4;
+        const size_t esz = prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+        while (W > 1 && (size_t)L * W * esz > 150 * 1024) W /= 2;
+        nparts = (size_t)(N + W - 1) / W;
+    }
+    return (size_t)L * 2 + 2 * nparts + nparts * (size_t)L + 8;
 }
 
 int launch_af(hipStream_t s, int prec, int L, int N, const void* rd, int64_t ld, const double* phase, const DevFft& fl,
-              void* img, double* work, int want_grad, double* grad, double* scal) {
-    if (prec == S1G_F32) return run_af<float>(s, L, N, rd, ld, phase, fl, img, work, want_grad, grad, scal);
-    return run_af<double>(s, L, N, rd, ld, phase, fl, img, work, want_grad, grad, scal);
+              void* img, void* img_tmp, double* work, int want_grad, double* grad, double* scal) {
+    if (prec == S1G_F32) return run_af<float>(s, L, N, rd, ld, phase, fl, img, img_tmp, work, want_grad, grad, scal);
+    return run_af<double>(s, L, N, rd, ld, phase, fl, img, img_tmp, work, want_grad, grad, scal);
 }
 
 }  // namespace s1g

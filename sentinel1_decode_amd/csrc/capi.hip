@@ -1321,19 +1321,20 @@ static int af_run(s1g_ctx* c, int prec, int L, int N, const void* rd, int64_t ld
     const DevFft* fl;
     int rc = get_plan(c, L, &fl);
     if (rc) return rc;
-    const size_t img_bytes = esize(prec) * (size_t)L * N;
-    const size_t work = sizeof(double) * (af_work_doubles(L, N, prec) + 4);
-    rc = ensure(c, &c->af_img, &c->af_img_bytes, img_bytes);
+    const bool fast = af_fast(L, N, ld, prec);
+    const size_t work = sizeof(double) * (af_work_doubles(L, N, ld, prec) + 4);
+    if (!fast) rc = ensure(c, &c->af_img, &c->af_img_bytes, esize(prec) * (size_t)L * N);
     if (!rc) rc = ensure(c, (void**)&c->af_scalars, &c->af_n, work);
     if (rc) return rc;
     double* scal = c->af_scalars;
     double* wk = c->af_scalars + 4;
     {
         StageTimer tm(c, S1G_STAGE_AUTOFOCUS);
-        rc = launch_af(c->stream, prec, L, N, rd, ld, phase, *fl, c->af_img, wk, grad != nullptr, grad, scal);
+        rc = launch_af(c->stream, prec, L, N, rd, ld, phase, *fl, fast ? img : nullptr, c->af_img, wk,
+                       grad != nullptr, grad, scal);
     }
     if (rc) return check_launch(c, rc, "autofocus");
-    if (img) {
+    if (img && !fast) {
         const size_t row = esize(prec) * (size_t)N;
         HIPCHK(c, hipMemcpy2DAsync(img, row, c->af_img, row, row, L, hipMemcpyDeviceToDevice, c->stream));
     }

@@ -143,11 +143,22 @@ int launch_replica_chirp(hipStream_t stream, int prec, void* out, int len, int n
                          double txpl);
 int launch_sinc_interp(hipStream_t stream, int prec, const void* x, int len, const double* t, int m, int L, void* out);
 
-// Autofocus.  work: af_work_doubles() doubles; scal: 3 doubles (S, T, E);
-// img: n_lines * n_samples complex (dense).
-size_t af_work_doubles(int n_lines, int n_samples, int prec);
+// Autofocus.  work: af_work_doubles() doubles; scal: 3 doubles (S, T, E).
+// img (may be null): the image, n_lines * n_samples complex (dense).
+// Lengths with compiled radices (af_fast) recompute the image from rd in
+// each pass; other lengths keep it in img_tmp (n_lines * n_samples complex)
+// between the entropy and gradient kernels.
+bool af_fast(int n_lines, int n_samples, int64_t ld, int prec);
+size_t af_work_doubles(int n_lines, int n_samples, int64_t ld, int prec);
+// the compiled-radix passes (autofocus_reg.hip): pass 0 entropy partials
+// (and the image when img is set), pass 2 gradient partials; one partial
+// per workgroup, af_reg_workgroups() of them
+int af_reg_workgroups(int n_samples, int prec);
+int launch_af_reg(hipStream_t stream, int prec, int pass, int n_lines, int n_samples, const void* rd, int64_t ld,
+                  const DevFft& fl, const void* ph, const double* scal, void* img, double* part, double* gpart);
+int launch_af_sum_partials(hipStream_t stream, const double* gpart, int nparts, int n_lines, double* grad);
 int launch_af(hipStream_t stream, int prec, int n_lines, int n_samples, const void* rd, int64_t ld,
-              const double* phase, const DevFft& fl, void* img, double* work, int want_grad, double* grad,
-              double* scal);
+              const double* phase, const DevFft& fl, void* img, void* img_tmp, double* work, int want_grad,
+              double* grad, double* scal);
 
 }  // namespace s1g

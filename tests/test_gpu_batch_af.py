@@ -137,3 +137,28 @@ def test_af_finite_differences(ctx, prec, dt):
         fd = (e1 - e2) / (2 * eps)
         tol = 1e-5 if prec == F64 else 1e-3
         assert abs(fd - g[i]) <= tol * scale, (i, fd, g[i])
+
+
+@pytest.mark.parametrize("L,N,wide", [(256, 99, 0), (1024, 37, 5), (300, 40, 3)])
+@pytest.mark.parametrize("prec,dt", [(F64, torch.complex128), (F32, torch.complex64)])
+def test_af_shapes_vs_oracle(ctx, L, N, wide, prec, dt):
+    """Lengths with compiled radices (256, 1024; a partial last column panel,
+    a row stride wider than the row) and one without (300: the generic
+    kernels that keep the image between the passes): entropy and gradient
+    against the oracle, the image output against numpy."""
+    rd, phase = _af_problem(L=L, N=N, seed=L + N)
+    e_ref, g_ref = oracle.af_entropy_grad(rd, phase)
+    full = torch.zeros((L, N + wide), dtype=dt, device="cuda")
+    full[:, :N] = torch.from_numpy(rd).to(dt).cuda()
+    d_rd = full[:, :N]
+    d_ph = torch.from_numpy(phase).cuda()
+    g = torch.zeros(L, dtype=torch.float64, device="cuda")
+    e = ctx.af_adjoint(d_rd, d_ph, g)
+    f64 = prec == F64
+    assert e == pytest.approx(e_ref, rel=1e-12 if f64 else 1e-6)
+    assert rel_l2(g.cpu().numpy(), g_ref) < (1e-9 if f64 else 1e-4)
+    img = torch.empty((L, N), dtype=dt, device="cuda")
+    e2 = ctx.af_forward(d_rd, d_ph, img)
+    assert e2 == pytest.approx(e, rel=1e-12 if f64 else 1e-6)
+    ref_img = np.fft.ifft(rd * np.exp(1j * phase)[:, None], axis=0)
+    assert rel_l2(img.cpu().numpy(), ref_img) < (1e-12 if f64 else 1e-6)
