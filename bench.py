@@ -227,8 +227,11 @@ def _timed(fn, steps, warmup):
 def run_iw_batch(steps, warmup):
     """27 uneven bursts (9 IW1-like 1420 x 21600, 9 IW2-like 1480 x 24000,
     9 IW3-like 1520 x 20400; 200 Rayleigh targets each, clutter 0.1) through
-    the FP32 chain: one captured hipGraph for the whole batch against
-    launching the bursts one after another."""
+    the FP32 chain: one captured hipGraph of the batch (the bursts' chains in
+    turn, each decode beside the previous burst's focusing) against the
+    graph of the stage-ordered batch (one launch per stage and shape class
+    over per-burst descriptors) and against launching the bursts one after
+    another."""
     import torch
     from sentinel1_decode_amd import capi, pipeline, synth
     t0 = time.perf_counter()
@@ -241,6 +244,7 @@ def run_iw_batch(steps, warmup):
     descs = [db.burst_desc(o) for db, o in zip(dbs, outs)]
     samples = sum(db.n_lines * db.n_samples for db in dbs)
     ms_graph = _timed(lambda: ctx.focus_batch(descs, capi.S1G_F32, graph=True), steps, warmup)
+    ms_staged = _timed(lambda: ctx.focus_batch(descs, capi.S1G_F32, graph=True, staged=True), steps, warmup)
     ms_seq = _timed(lambda: [db.focus(ctx, capi.S1G_F32, out=o) for db, o in zip(dbs, outs)], steps, warmup)
     failed = sum(int((db.d_status.cpu().numpy() != 0).sum()) for db in dbs)
     ctx.close()
@@ -250,6 +254,8 @@ def run_iw_batch(steps, warmup):
             "dtype": "f32", "data": "synthetic",
             "config": {"workload": "27 bursts: 9x 1420x21600 (fs 50.05 MHz, PRF 1685), 9x 1480x24000 "
                                    "(64.35 MHz, 1717), 9x 1520x20400 (54.60 MHz, 1451)", "samples": samples},
+            "staged_graph_ms_per_step": round(ms_staged, 4),
+            "staged_graph_msamples_s": round(samples / (ms_staged * 1e-3) / 1e6, 3),
             "sequential_ms_per_step": round(ms_seq, 4),
             "sequential_msamples_s": round(samples / (ms_seq * 1e-3) / 1e6, 3),
             "graph_speedup": round(ms_seq / ms_graph, 3), "packets_failed": failed}

@@ -442,16 +442,29 @@ typedef struct {
 
 int s1g_focus_burst(s1g_ctx* ctx, const s1g_burst_desc* desc, int prec);
 
-/* Several bursts (uneven sizes allowed) in one submission.  flags:
- *   S1G_BATCH_GRAPH  capture the whole batch as one hipGraph and replay it
- *                    (the graph is cached per descriptor set).
- * When no burst uses ECCM and no two bursts' SLC rows or status words
- * overlap, the decode of burst i+1 runs on a second stream of the context
- * beside the focusing of burst i (each burst's focusing waits only on its
- * own decode); otherwise the chains run in order.  Either way the batch is
- * ordered on the context's stream as one unit.  A burst's SLC must not
- * alias another burst's packet data.                                       */
+/* Several bursts (uneven sizes allowed) in one submission
+ * (_get_azimuth_compressed_swath_iw's burst loop, s1_decoder.cpp:307-349).
+ * flags:
+ *   S1G_BATCH_GRAPH   capture the whole batch as one hipGraph and replay it
+ *                     (the graph is cached per descriptor set);
+ *   S1G_BATCH_STAGED  run the batch stage by stage: every decode, then range
+ *                     compression, the azimuth FFT, RCMC + matched filter
+ *                     and the azimuth IFFT each as one launch over all
+ *                     bursts of a shape class (the persistent kernels take a
+ *                     table of per-burst descriptors), with a workspace of
+ *                     the batch's total size in the context.  Needs no ECCM
+ *                     and disjoint outputs (else the default order is used).
+ * By default the chains run burst by burst; when no burst uses ECCM and no
+ * two bursts' SLC rows or status words overlap, the decode of burst i+1
+ * runs on a second stream of the context beside the focusing of burst i
+ * (each burst's focusing waits only on its own decode).  Burst by burst is
+ * the faster order on MI355X: a burst's intermediate array (284 MB for
+ * IW2 FP32) is read back by the next stage while it is still in the
+ * 256 MB last-level cache, which a stage over all bursts gives up.  Either
+ * way the batch is ordered on the context's stream as one unit.  A burst's
+ * SLC must not alias another burst's packet data. */
 #define S1G_BATCH_GRAPH 1
+#define S1G_BATCH_STAGED 2
 int s1g_focus_batch(s1g_ctx* ctx, const s1g_burst_desc* descs, int n_bursts, int prec, int flags);
 
 /* ------------------------------------------- general signal processing */

@@ -18,6 +18,7 @@ S1G_F64 = 0
 S1G_F32 = 1
 S1G_OK = 0
 S1G_BATCH_GRAPH = 1
+S1G_BATCH_STAGED = 2
 PKT_STATUS = {0: "ok", 1: "brc", 2: "overrun", 3: "huffman", 4: "mcode", 5: "arg"}
 
 
@@ -595,11 +596,14 @@ class Context:
         self._bind_stream()
         self._check(lib().s1g_focus_burst(self.h, C.byref(desc), prec), "focus_burst")
 
-    def focus_batch(self, descs, prec, graph=False):
+    def focus_batch(self, descs, prec, graph=False, staged=False):
+        """s1g_focus_batch: the bursts' chains one after another (default),
+        or with staged one launch per stage over all bursts (no ECCM,
+        disjoint outputs); graph captures and replays."""
         arr = (BurstDesc * len(descs))(*descs)
+        flags = (S1G_BATCH_GRAPH if graph else 0) | (S1G_BATCH_STAGED if staged else 0)
         self._bind_stream()
-        self._check(lib().s1g_focus_batch(self.h, arr, len(descs), prec, S1G_BATCH_GRAPH if graph else 0),
-                    "focus_batch")
+        self._check(lib().s1g_focus_batch(self.h, arr, len(descs), prec, flags), "focus_batch")
 
     # ------------------------------------------------------------- autofocus
     def af_forward(self, rd, phase, img=None):

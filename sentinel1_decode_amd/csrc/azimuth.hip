@@ -5,6 +5,7 @@
 // signal_processing.cpp:581-605) folded in.
 #include "s1g_internal.h"
 
+#include <algorithm>
 #include <initializer_list>
 
 namespace s1g {
@@ -106,6 +107,27 @@ static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64
                        s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale, sh);
 }
 
+// A global panel index of a batch: its instance's buffer resource, leading
+// dimension, first column and width (w = 0 past the last panel, so every
+// access of it is masked).  Uniform: a scan of the kernel arguments.
+struct AzPanel {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int64_t ld;
+    int j0, w;
+};
+template <typename T, int W>
+__device__ __forceinline__ AzPanel az_panel(const AzBatch& bt, int L, int panel) {
+    int k = 0;
+    while (k + 1 < bt.n && panel >= bt.inst[k + 1].panel0) ++k;
+    const AzInst& I = bt.inst[k];
+    AzPanel p;
+    p.ld = I.ld;
+    p.j0 = (panel - I.panel0) * W;
+    p.w = panel < bt.npanels ? min(W, I.N - p.j0) : 0;
+    p.rsrc = buf_rsrc(I.data, (uint32_t)(((int64_t)(L - 1) * I.ld + I.N) * (int64_t)sizeof(cpx<T>)));
+    return p;
+}
+
 // Persistent form of az_fft_fixed_kernel: a panel fills most of a CU's LDS,
 // so workgroups cannot overlap one another's HBM traffic; instead one
 // workgroup per CU walks its panels and issues the next panel's loads into
@@ -117,9 +139,11 @@ static void launch_az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64
 // the wait for a panel's prefetched registers counts past the previous
 // panel's stores, which keep draining while the next panel is transformed.
 // The array must span less than 2 GiB (checked by the launcher).
+// One launch may serve several arrays of the same length (AzBatch: per
+// instance data, leading dimension, columns and first global panel); the
+// workgroups deal the panels of all of them.
 template <typename T, bool INV, int NT, int W, int MAXE, int... Rs>
-__global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ data, int64_t ld, int N, int npanels,
-                                                         const cpx<T>* __restrict__ tw_g,
+__global__ __launch_bounds__(NT) void az_fft_pipe_kernel(AzBatch bt, const cpx<T>* __restrict__ tw_g,
                                                          const int32_t* __restrict__ perm_g, T scale, int sh) {
     constexpr int L = (Rs * ...);
     constexpr int total = L * W;
@@ -131,8 +155,7 @@ __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ da
     const int tid0 = threadIdx.x;
     const int G = (int)gridDim.x;           // a multiple of 8
     const int first = (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
-    const uint32_t bytes = (uint32_t)(((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>));
-    const auto rsrc = buf_rsrc(data, bytes);
+    const int npanels = bt.npanels;
     for (int k = tid0; k < L; k += NT) {
         tw[k] = tw_g[k];
         perm[k] = perm_g[k];
@@ -141,8 +164,10 @@ __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ da
     // element offsets of a panel (kBufSkip past the array, the panel or the
     // last panel's columns)
     auto load = [&](int panel, int tid) {
-        const int j0 = panel * W;
-        const int w = panel < npanels ? min(W, N - j0) : 0;
+        const AzPanel P = az_panel<T, W>(bt, L, panel);
+        const int j0 = P.j0, w = P.w;
+        const int64_t ld = P.ld;
+        const auto rsrc = P.rsrc;
 #pragma unroll
         for (int i = 0; i < MAXE; ++i) {
             const int idx = tid + NT * i;
@@ -173,8 +198,10 @@ __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ da
         asm volatile("" : "+v"(tid));
         load(panel + G, tid);  // in flight during the passes (all masked past the last panel)
         DifSeq<T, INV, Rs...>::run(buf, L, 1, W, 1, W, tw, tid, NT);
-        const int j0 = panel * W;
-        const int w = min(W, N - j0);
+        const AzPanel P = az_panel<T, W>(bt, L, panel);
+        const int j0 = P.j0, w = P.w;
+        const int64_t ld = P.ld;
+        const auto rsrc = P.rsrc;
 #pragma unroll
         for (int i = 0; i < MAXE; ++i) {
             const int idx = tid + NT * i;
@@ -205,9 +232,9 @@ __global__ __launch_bounds__(NT) void az_fft_pipe_kernel(cpx<T>* __restrict__ da
 // and two barriers per panel fewer than az_fft_pipe_kernel.  Prefetch,
 // XCD-aware dealing and the counted waits (buffer accesses, masked by
 // out-of-range offsets) are as there; the array must span less than 2 GiB.
+// One launch may serve several arrays of the same length, as the pipe kernel.
 template <typename T, bool INV, int NT, int W, int RA, int RZ, int... MIDS>
-__global__ __launch_bounds__(NT) void az_fft_reg_kernel(cpx<T>* __restrict__ data, int64_t ld, int N, int npanels,
-                                                        const cpx<T>* __restrict__ tw_g,
+__global__ __launch_bounds__(NT) void az_fft_reg_kernel(AzBatch bt, const cpx<T>* __restrict__ tw_g,
                                                         const int32_t* __restrict__ perm_g, T scale, int sh) {
     constexpr int L = RA * (MIDS * ... * 1) * RZ;
     constexpr int LZ = L / RZ;                       // first-pass butterflies per column (their stride)
@@ -221,16 +248,17 @@ __global__ __launch_bounds__(NT) void az_fft_reg_kernel(cpx<T>* __restrict__ dat
     const int tid0 = threadIdx.x;
     const int G = (int)gridDim.x;           // a multiple of 8
     const int first = (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
-    const uint32_t bytes = (uint32_t)(((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>));
-    const auto rsrc = buf_rsrc(data, bytes);
+    const int npanels = bt.npanels;
     for (int k = tid0; k < L; k += NT) {
         tw[k] = tw_g[k];
         ipos[perm_g[k]] = k;
     }
     cpx<T> r[QF * RZ];
     auto load = [&](int panel, int tid) {
-        const int j0 = panel * W;
-        const int w = panel < npanels ? min(W, N - j0) : 0;
+        const AzPanel P = az_panel<T, W>(bt, L, panel);
+        const int j0 = P.j0, w = P.w;
+        const int64_t ld = P.ld;
+        const auto rsrc = P.rsrc;
 #pragma unroll
         for (int q = 0; q < QF; ++q) {
             const int idx = tid + NT * q;
@@ -269,8 +297,10 @@ __global__ __launch_bounds__(NT) void az_fft_reg_kernel(cpx<T>* __restrict__ dat
     };
     // last DIF pass (radix RA, Ns = 1) and the store
     auto last = [&](int panel, int tid) {
-        const int j0 = panel * W;
-        const int w = min(W, N - j0);
+        const AzPanel P = az_panel<T, W>(bt, L, panel);
+        const int j0 = P.j0, w = P.w;
+        const int64_t ld = P.ld;
+        const auto rsrc = P.rsrc;
 #pragma unroll
         for (int q = 0; q < QL; ++q) {
             const int idx = tid + NT * q;
@@ -315,20 +345,48 @@ static int pipe_workgroups(int npanels) {
     return std::min(g, ((npanels + 7) / 8) * 8);
 }
 
+// Every array of a batch spans less than 2 GiB (32-bit buffer offsets).
+template <typename T>
+static bool az_spans_ok(const AzInst* inst, int n, int L) {
+    for (int k = 0; k < n; ++k)
+        if (((int64_t)(L - 1) * inst[k].ld + inst[k].N) * (int64_t)sizeof(cpx<T>) >= ((int64_t)1 << 31)) return false;
+    return true;
+}
+
+// The instances of a persistent launch (at most kMaxBatch), panels numbered
+// across them; returns the panel count.
+template <int W>
+static int az_batch_of(const AzInst* inst, int n, AzBatch& bt) {
+    bt.n = n;
+    int np = 0;
+    for (int k = 0; k < n; ++k) {
+        bt.inst[k] = inst[k];
+        bt.inst[k].panel0 = np;
+        np += (inst[k].N + W - 1) / W;
+    }
+    bt.npanels = np;
+    return np;
+}
+
 template <typename T, bool INV, int W, int... Rs>
-static void launch_az_pipe(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
+static void launch_az_pipe(hipStream_t s, const DevFft& fl, const AzInst* inst, int n, T scale, int sh) {
     constexpr int NT = 512;
     constexpr int L = (Rs * ...);
-    if (((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>) >= ((int64_t)1 << 31)) {
+    if (!az_spans_ok<T>(inst, n, L)) {
         // 32-bit buffer offsets: larger arrays take the one-panel kernel
-        launch_az_fixed<T, INV, W, Rs...>(s, fl, data, ld, N, scale, sh);
+        for (int k = 0; k < n; ++k)
+            launch_az_fixed<T, INV, W, Rs...>(s, fl, (cpx<T>*)inst[k].data, inst[k].ld, inst[k].N, scale, sh);
         return;
     }
     constexpr int MAXE = (L * W + NT - 1) / NT;
-    const int npanels = (N + W - 1) / W;
     const size_t lds = (size_t)L * W * sizeof(cpx<T>) + (size_t)L * (sizeof(cpx<T>) + sizeof(int32_t));
-    hipLaunchKernelGGL((az_fft_pipe_kernel<T, INV, NT, W, MAXE, Rs...>), dim3((unsigned)pipe_workgroups(npanels)),
-                       dim3(NT), lds, s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale, sh);
+    for (int b0 = 0; b0 < n; b0 += kMaxBatch) {
+        AzBatch bt;
+        const int np = az_batch_of<W>(inst + b0, std::min(kMaxBatch, n - b0), bt);
+        if (np == 0) continue;
+        hipLaunchKernelGGL((az_fft_pipe_kernel<T, INV, NT, W, MAXE, Rs...>), dim3((unsigned)pipe_workgroups(np)),
+                           dim3(NT), lds, s, bt, tw_of<T>(fl), fl.perm, scale, sh);
+    }
 }
 
 // Whether a plan's radix sequence is one compiled into az_fft_fixed_kernel.
@@ -347,48 +405,54 @@ static int fixed_sequence(const FftPlan& p) {
 }
 
 template <typename T, bool INV, int W, int RA, int RZ, int... MIDS>
-static void launch_az_reg(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
+static void launch_az_reg(hipStream_t s, const DevFft& fl, const AzInst* inst, int n, T scale, int sh) {
     constexpr int NT = 512;
     constexpr int L = RA * (MIDS * ... * 1) * RZ;
-    if (((int64_t)(L - 1) * ld + N) * (int64_t)sizeof(cpx<T>) >= ((int64_t)1 << 31)) {
+    if (!az_spans_ok<T>(inst, n, L)) {
         // 32-bit buffer offsets: larger arrays take the one-panel kernel
-        launch_az_fixed<T, INV, W, RA, MIDS..., RZ>(s, fl, data, ld, N, scale, sh);
+        for (int k = 0; k < n; ++k)
+            launch_az_fixed<T, INV, W, RA, MIDS..., RZ>(s, fl, (cpx<T>*)inst[k].data, inst[k].ld, inst[k].N, scale,
+                                                        sh);
         return;
     }
-    const int npanels = (N + W - 1) / W;
     const size_t lds = (size_t)L * W * sizeof(cpx<T>) + (size_t)L * (sizeof(cpx<T>) + sizeof(int32_t));
-    hipLaunchKernelGGL((az_fft_reg_kernel<T, INV, NT, W, RA, RZ, MIDS...>), dim3((unsigned)pipe_workgroups(npanels)),
-                       dim3(NT), lds, s, data, ld, N, npanels, tw_of<T>(fl), fl.perm, scale, sh);
+    for (int b0 = 0; b0 < n; b0 += kMaxBatch) {
+        AzBatch bt;
+        const int np = az_batch_of<W>(inst + b0, std::min(kMaxBatch, n - b0), bt);
+        if (np == 0) continue;
+        hipLaunchKernelGGL((az_fft_reg_kernel<T, INV, NT, W, RA, RZ, MIDS...>), dim3((unsigned)pipe_workgroups(np)),
+                           dim3(NT), lds, s, bt, tw_of<T>(fl), fl.perm, scale, sh);
+    }
 }
 
 template <typename T, bool INV, int W>
-static bool az_reg(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
+static bool az_reg(hipStream_t s, const DevFft& fl, const AzInst* inst, int n, T scale, int sh) {
     switch (fixed_sequence(fl.plan)) {
-    case 1: launch_az_reg<T, INV, W, 37, 8, 5>(s, fl, data, ld, N, scale, sh); return true;
-    case 2: launch_az_reg<T, INV, W, 71, 4, 5>(s, fl, data, ld, N, scale, sh); return true;
-    case 3: launch_az_reg<T, INV, W, 19, 2, 5, 8>(s, fl, data, ld, N, scale, sh); return true;
+    case 1: launch_az_reg<T, INV, W, 37, 8, 5>(s, fl, inst, n, scale, sh); return true;
+    case 2: launch_az_reg<T, INV, W, 71, 4, 5>(s, fl, inst, n, scale, sh); return true;
+    case 3: launch_az_reg<T, INV, W, 19, 2, 5, 8>(s, fl, inst, n, scale, sh); return true;
+    default: return false;
+    }
+}
+
+template <typename T, bool INV, int W>
+static bool az_pipe(hipStream_t s, const DevFft& fl, const AzInst* inst, int n, T scale, int sh) {
+    switch (fixed_sequence(fl.plan)) {
+    case 1: launch_az_pipe<T, INV, W, 37, 5, 8>(s, fl, inst, n, scale, sh); return true;
+    case 2: launch_az_pipe<T, INV, W, 71, 5, 4>(s, fl, inst, n, scale, sh); return true;
+    case 3: launch_az_pipe<T, INV, W, 19, 5, 8, 2>(s, fl, inst, n, scale, sh); return true;
     default: return false;
     }
 }
 
 // Plans with a compiled radix sequence (the IW1/IW2/IW3 burst lengths).
-template <typename T, bool INV, int W, bool PIPE>
+template <typename T, bool INV, int W>
 static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, int N, T scale, int sh) {
     switch (fixed_sequence(fl.plan)) {
-    case 1:
-        if constexpr (PIPE) launch_az_pipe<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale, sh);
-        else launch_az_fixed<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale, sh);
-        return true;
-    case 2:
-        if constexpr (PIPE) launch_az_pipe<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale, sh);
-        else launch_az_fixed<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale, sh);
-        return true;
-    case 3:
-        if constexpr (PIPE) launch_az_pipe<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale, sh);
-        else launch_az_fixed<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale, sh);
-        return true;
-    default:
-        return false;
+    case 1: launch_az_fixed<T, INV, W, 37, 5, 8>(s, fl, data, ld, N, scale, sh); return true;
+    case 2: launch_az_fixed<T, INV, W, 71, 5, 4>(s, fl, data, ld, N, scale, sh); return true;
+    case 3: launch_az_fixed<T, INV, W, 19, 5, 8, 2>(s, fl, data, ld, N, scale, sh); return true;
+    default: return false;
     }
 }
 
@@ -475,40 +539,62 @@ int azimuth_fft_variants(const DevFft& fl) { return fixed_sequence(fl.plan) ? 5 
 // panels, persistent with the next panel's loads overlapping the passes,
 // 4 = as 3 with the first and last passes in registers (az_fft_reg_kernel);
 // otherwise only the generic kernel.
+// The persistent variants (3, 4) for a list of arrays of length fl.plan.n.
+template <typename T>
+static void az_persistent(hipStream_t s, const DevFft& fl, const AzInst* inst, int n, int inverse, int variant,
+                          int sh) {
+    constexpr int W = sizeof(T) == 4 ? 8 : 4;
+    const T sc = (T)(1.0 / (double)fl.plan.n);
+    if (variant == 4)
+        inverse ? az_reg<T, true, W>(s, fl, inst, n, sc, sh) : az_reg<T, false, W>(s, fl, inst, n, (T)1, sh);
+    else
+        inverse ? az_pipe<T, true, W>(s, fl, inst, n, sc, sh) : az_pipe<T, false, W>(s, fl, inst, n, (T)1, sh);
+}
+
+bool azimuth_batchable(const DevFft& fl, int variant) {
+    return fixed_sequence(fl.plan) != 0 && (variant == 3 || variant == 4);
+}
+
+int launch_azimuth_fft_batch(hipStream_t s, int prec, const DevFft& fl, const AzInst* inst, int n, int inverse,
+                             int variant) {
+    if (!azimuth_batchable(fl, variant)) return -1;
+    const int L = fl.plan.n;
+    const int sh = inverse ? 0 : L - L / 2;  // the chain's fftshift after the forward transform
+    if (prec == S1G_F32) az_persistent<float>(s, fl, inst, n, inverse, variant, sh);
+    else az_persistent<double>(s, fl, inst, n, inverse, variant, sh);
+    return (int)hipGetLastError();
+}
+
 int launch_azimuth_fft(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
                        int inverse, int variant, int shift) {
     const int sh = (!inverse && shift) ? L - L / 2 : 0;
     if (fixed_sequence(fl.plan) && variant != 2) {
-        if (prec == S1G_F32) {
+        AzInst one;
+        one.data = data;
+        one.ld = ld;
+        one.N = N;
+        one.panel0 = 0;
+        if (variant == 3 || variant == 4) {
+            if (prec == S1G_F32) az_persistent<float>(s, fl, &one, 1, inverse, variant, sh);
+            else az_persistent<double>(s, fl, &one, 1, inverse, variant, sh);
+        } else if (prec == S1G_F32) {
             const float sc = (float)(1.0 / (double)L);
             cpx<float>* d = (cpx<float>*)data;
-            if (variant == 4)
-                inverse ? az_reg<float, true, 8>(s, fl, d, ld, N, sc, sh)
-                        : az_reg<float, false, 8>(s, fl, d, ld, N, 1.0f, sh);
-            else if (variant == 3)
-                inverse ? az_fixed<float, true, 8, true>(s, fl, d, ld, N, sc, sh)
-                        : az_fixed<float, false, 8, true>(s, fl, d, ld, N, 1.0f, sh);
-            else if (variant == 1)
-                inverse ? az_fixed<float, true, 4, false>(s, fl, d, ld, N, sc, sh)
-                        : az_fixed<float, false, 4, false>(s, fl, d, ld, N, 1.0f, sh);
+            if (variant == 1)
+                inverse ? az_fixed<float, true, 4>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<float, false, 4>(s, fl, d, ld, N, 1.0f, sh);
             else
-                inverse ? az_fixed<float, true, 8, false>(s, fl, d, ld, N, sc, sh)
-                        : az_fixed<float, false, 8, false>(s, fl, d, ld, N, 1.0f, sh);
+                inverse ? az_fixed<float, true, 8>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<float, false, 8>(s, fl, d, ld, N, 1.0f, sh);
         } else {
             const double sc = 1.0 / (double)L;
             cpx<double>* d = (cpx<double>*)data;
-            if (variant == 4)
-                inverse ? az_reg<double, true, 4>(s, fl, d, ld, N, sc, sh)
-                        : az_reg<double, false, 4>(s, fl, d, ld, N, 1.0, sh);
-            else if (variant == 3)
-                inverse ? az_fixed<double, true, 4, true>(s, fl, d, ld, N, sc, sh)
-                        : az_fixed<double, false, 4, true>(s, fl, d, ld, N, 1.0, sh);
-            else if (variant == 1)
-                inverse ? az_fixed<double, true, 2, false>(s, fl, d, ld, N, sc, sh)
-                        : az_fixed<double, false, 2, false>(s, fl, d, ld, N, 1.0, sh);
+            if (variant == 1)
+                inverse ? az_fixed<double, true, 2>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<double, false, 2>(s, fl, d, ld, N, 1.0, sh);
             else
-                inverse ? az_fixed<double, true, 4, false>(s, fl, d, ld, N, sc, sh)
-                        : az_fixed<double, false, 4, false>(s, fl, d, ld, N, 1.0, sh);
+                inverse ? az_fixed<double, true, 4>(s, fl, d, ld, N, sc, sh)
+                        : az_fixed<double, false, 4>(s, fl, d, ld, N, 1.0, sh);
         }
         return (int)hipGetLastError();
     }

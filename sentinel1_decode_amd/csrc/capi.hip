@@ -355,9 +355,10 @@ int check_launch(s1g_ctx* c, int rc, const char* what) {
     return 0;
 }
 
-int do_range(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int64_t ld_in, void* out,
-             int64_t ld_out) {
-    if (d.n_lines <= 0 || d.n_samples <= 0) return d.n_lines < 0 || d.n_samples < 0 ? fail(c, S1G_EARG, "bad range shape%s") : 0;
+// The range stage's block length (forced, cached or timed now on these
+// buffers) with its plan and filter; *nc_out the replica length.
+static int range_setup(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int64_t ld_in, void* out,
+                       int64_t ld_out, const DevFft** fm_out, void** h_out, int* nc_out) {
     const int nc = (int)floor(d.fs_mhz * d.txpl_us);
     if (nc < 2) return fail(c, S1G_EARG, "replica chirp shorter than 2 samples%s");
     int cand[2];
@@ -378,26 +379,32 @@ int do_range(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int6
                             "range compression");
     };
     const int v = choose_variant(c, TuneKey{S1G_STAGE_RANGE, prec, d.n_lines, d.n_samples, nc}, nvar, run);
+    *nc_out = nc;
+    StageTimer tm(c, S1G_STAGE_FILTER);
+    return prepare(v, fm_out, h_out);
+}
+
+int do_range(s1g_ctx* c, const s1g_range_desc& d, int prec, const void* in, int64_t ld_in, void* out,
+             int64_t ld_out) {
+    if (d.n_lines <= 0 || d.n_samples <= 0) return d.n_lines < 0 || d.n_samples < 0 ? fail(c, S1G_EARG, "bad range shape%s") : 0;
     const DevFft* fm;
     void* h;
-    int rc;
-    {
-        StageTimer tm(c, S1G_STAGE_FILTER);
-        rc = prepare(v, &fm, &h);
-    }
+    int nc;
+    int rc = range_setup(c, d, prec, in, ld_in, out, ld_out, &fm, &h, &nc);
     if (rc) return rc;
     StageTimer tm(c, S1G_STAGE_RANGE);
     return check_launch(c, launch_range_os(c->stream, prec, *fm, in, ld_in, out, ld_out, d.n_lines, d.n_samples, nc, h),
                         "range compression");
 }
 
-int do_azimuth_fft(s1g_ctx* c, int prec, int L, int N, void* data, int64_t ld, int inverse) {
-    if (L <= 0 || N <= 0) return 0;
+// The azimuth transform's plan and launch variant for an L x N array
+// (forced, cached, or timed now on a scratch array: the transform is in
+// place).
+static int az_setup(s1g_ctx* c, int prec, int L, int N, int inverse, const DevFft** fl_out, int* v_out) {
     const DevFft* fl;
     int rc = get_plan(c, L, &fl);
     if (rc) return rc;
     const int stage = inverse ? S1G_STAGE_AZ_IFFT : S1G_STAGE_AZ_FFT;
-    // the transform is in place: candidates are timed on a scratch array
     void* tmp = nullptr;
     auto run = [&](int v) {
         if (!tmp) {
@@ -414,26 +421,49 @@ int do_azimuth_fft(s1g_ctx* c, int prec, int L, int N, void* data, int64_t ld, i
         (void)hipStreamSynchronize(c->stream);
         hipFree(tmp);
     }
-    StageTimer tm(c, stage);
+    *fl_out = fl;
+    *v_out = v;
+    return 0;
+}
+
+int do_azimuth_fft(s1g_ctx* c, int prec, int L, int N, void* data, int64_t ld, int inverse) {
+    if (L <= 0 || N <= 0) return 0;
+    const DevFft* fl;
+    int v;
+    int rc = az_setup(c, prec, L, N, inverse, &fl, &v);
+    if (rc) return rc;
+    StageTimer tm(c, inverse ? S1G_STAGE_AZ_IFFT : S1G_STAGE_AZ_FFT);
     return check_launch(c, launch_azimuth_fft(c->stream, prec, *fl, data, ld, L, N, inverse, v), "azimuth fft");
+}
+
+// The column table of d in cols, and the RCMC stage's launch variant
+// (forced, cached or timed now on these buffers).
+static int rcmc_setup(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, AzColumn* cols, const void* in,
+                      int64_t ld_in, void* out, int64_t ld_out, int* v_out) {
+    if (d.n_lines <= 1 || d.n_samples <= 0) return fail(c, S1G_EARG, "azimuth stage needs >= 2 lines%s");
+    if (!d.slant_ranges || !d.v_rel || !d.f_dc || !d.t_corr) return fail(c, S1G_EARG, "null geometry array%s");
+    int rc = launch_az_columns(c->stream, d, cols);
+    if (rc) return hip_fail(c, (hipError_t)rc, "azimuth columns");
+    auto run = [&](int v) {
+        return check_launch(c, launch_rcmc_mf(c->stream, prec, d, cols, in, ld_in, out, ld_out, v),
+                            "rcmc/matched filter");
+    };
+    *v_out = choose_variant(c, TuneKey{S1G_STAGE_RCMC_MF, prec, d.n_lines, d.n_samples, d.apply_rcmc},
+                            rcmc_variants(prec), run);
+    return 0;
 }
 
 int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int64_t ld_in, void* out,
             int64_t ld_out) {
     if (d.n_lines <= 1 || d.n_samples <= 0) return fail(c, S1G_EARG, "azimuth stage needs >= 2 lines%s");
-    if (!d.slant_ranges || !d.v_rel || !d.f_dc || !d.t_corr) return fail(c, S1G_EARG, "null geometry array%s");
     int rc = ensure(c, (void**)&c->cols, &c->cols_n, az_table_bytes(d.n_samples));
     if (rc) return rc;
-    rc = launch_az_columns(c->stream, d, c->cols);
-    if (rc) return hip_fail(c, (hipError_t)rc, "azimuth columns");
-    auto run = [&](int v) {
-        return check_launch(c, launch_rcmc_mf(c->stream, prec, d, c->cols, in, ld_in, out, ld_out, v),
-                            "rcmc/matched filter");
-    };
-    const int v = choose_variant(c, TuneKey{S1G_STAGE_RCMC_MF, prec, d.n_lines, d.n_samples, d.apply_rcmc},
-                                 rcmc_variants(prec), run);
+    int v;
+    rc = rcmc_setup(c, d, prec, c->cols, in, ld_in, out, ld_out, &v);
+    if (rc) return rc;
     StageTimer tm(c, S1G_STAGE_RCMC_MF);
-    return run(v);
+    return check_launch(c, launch_rcmc_mf(c->stream, prec, d, c->cols, in, ld_in, out, ld_out, v),
+                        "rcmc/matched filter");
 }
 
 int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_t ld_in, void* out, int64_t ld_out,
@@ -1205,6 +1235,180 @@ static int batch_chains(s1g_ctx* c, const s1g_burst_desc* d, int n, int prec) {
     return S1G_OK;  // c->stream already waits on the last decode
 }
 
+// Bytes of burst i's workspace slice and column table in a stage-ordered
+// batch (256-byte aligned).
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+static void staged_layout(const s1g_burst_desc* d, int n, int prec, std::vector<size_t>& ws_off,
+                          std::vector<size_t>& col_off, size_t* ws_total, size_t* col_total) {
+    ws_off.resize(n);
+    col_off.resize(n);
+    size_t w = 0, k = 0;
+    for (int i = 0; i < n; ++i) {
+        ws_off[i] = w;
+        col_off[i] = k;
+        w += align256(esize(prec) * (size_t)d[i].range.n_lines * (size_t)d[i].range.n_samples);
+        k += align256(az_table_bytes(d[i].range.n_samples));
+    }
+    *ws_total = w;
+    *col_total = k;
+}
+
+// Stage-ordered batch (S1G_BATCH_STAGED): every burst's decode, then each
+// later stage as one launch over all bursts -- the persistent,
+// descriptor-driven kernels take a table of instances (range compression
+// per block length, the azimuth transforms per azimuth length and variant,
+// RCMC + matched filter per variant); a stage whose chosen variant has no
+// batched form runs burst by burst in its place.  Each burst keeps its own workspace slice (c->ws holds
+// them all) and column table (c->cols).  Callers check batch_can_overlap
+// (no ECCM, disjoint outputs) and size c->ws and c->cols by staged_layout.
+static int batch_staged(s1g_ctx* c, const s1g_burst_desc* d, int n, int prec) {
+    std::vector<size_t> wso, colo;
+    size_t ws_total, col_total;
+    staged_layout(d, n, prec, wso, colo, &ws_total, &col_total);
+    if (c->ws_bytes < ws_total || c->cols_n < col_total) return fail(c, S1G_EARG, "batch workspace not prepared%s");
+    auto ws_of = [&](int i) { return (void*)((char*)c->ws + wso[i]); };
+    auto cols_of = [&](int i) { return (AzColumn*)((char*)c->cols + colo[i]); };
+    for (int i = 0; i < n; ++i) {
+        const int L = d[i].range.n_lines, N = d[i].range.n_samples;
+        if (L <= 1 || N <= 0) return fail(c, S1G_EARG, "burst needs >= 2 lines%s");
+        if (d[i].azimuth.n_lines != L || d[i].azimuth.n_samples != N)
+            return fail(c, S1G_EARG, "descriptor shapes disagree%s");
+        if (d[i].ld_out < N || !d[i].out) return fail(c, S1G_EARG, "bad output buffer%s");
+    }
+    int rc = 0;
+    // decode, burst by burst (one workgroup per packet already fills the chip)
+    for (int i = 0; i < n && !rc; ++i) rc = burst_decode(c, d[i], prec);
+    if (rc) return rc;
+    // range compression: out -> workspace, grouped by block length
+    {
+        std::vector<const DevFft*> fm(n);
+        std::vector<RangeInst> inst(n);
+        for (int i = 0; i < n && !rc; ++i) {
+            void* h;
+            int nc;
+            const int N = d[i].range.n_samples;
+            rc = range_setup(c, d[i].range, prec, d[i].out, d[i].ld_out, ws_of(i), N, &fm[i], &h, &nc);
+            if (!rc) inst[i] = range_inst(d[i].out, d[i].ld_out, ws_of(i), N, d[i].range.n_lines, N, fm[i]->plan.n, nc, h);
+        }
+        if (rc) return rc;
+        StageTimer tm(c, S1G_STAGE_RANGE);
+        std::vector<bool> done(n, false);
+        for (int i = 0; i < n && !rc; ++i) {
+            if (done[i]) continue;
+            std::vector<RangeInst> group;
+            for (int k = i; k < n; ++k)
+                if (!done[k] && fm[k] == fm[i]) {
+                    group.push_back(inst[k]);
+                    done[k] = true;
+                }
+            if (range_batchable(prec, fm[i]->plan.n)) {
+                rc = check_launch(c, launch_range_os_batch(c->stream, prec, *fm[i], group.data(), (int)group.size()),
+                                  "range compression");
+            } else {
+                for (const RangeInst& r : group) {
+                    if (rc) break;
+                    const int nc = fm[i]->plan.n - r.V + 1;
+                    rc = check_launch(c, launch_range_os(c->stream, prec, *fm[i], r.in, r.ld_in, r.out, r.ld_out,
+                                                         r.n_lines, r.N, nc, r.h),
+                                      "range compression");
+                }
+            }
+        }
+        if (rc) return rc;
+    }
+    // the azimuth transforms (in place), grouped by length and variant
+    auto azimuth = [&](int inverse, auto data_of, auto ld_of) -> int {
+        std::vector<const DevFft*> fl(n);
+        std::vector<int> var(n);
+        int r = 0;
+        for (int i = 0; i < n && !r; ++i)
+            r = az_setup(c, prec, d[i].range.n_lines, d[i].range.n_samples, inverse, &fl[i], &var[i]);
+        if (r) return r;
+        StageTimer tm(c, inverse ? S1G_STAGE_AZ_IFFT : S1G_STAGE_AZ_FFT);
+        std::vector<bool> done(n, false);
+        for (int i = 0; i < n && !r; ++i) {
+            if (done[i]) continue;
+            std::vector<AzInst> group;
+            for (int k = i; k < n; ++k)
+                if (!done[k] && fl[k] == fl[i] && var[k] == var[i]) {
+                    AzInst a;
+                    a.data = data_of(k);
+                    a.ld = ld_of(k);
+                    a.N = d[k].range.n_samples;
+                    a.panel0 = 0;
+                    group.push_back(a);
+                    done[k] = true;
+                }
+            if (azimuth_batchable(*fl[i], var[i])) {
+                r = check_launch(c, launch_azimuth_fft_batch(c->stream, prec, *fl[i], group.data(), (int)group.size(),
+                                                             inverse, var[i]),
+                                 "azimuth fft");
+            } else {
+                for (const AzInst& a : group) {
+                    if (r) break;
+                    r = check_launch(c, launch_azimuth_fft(c->stream, prec, *fl[i], a.data, a.ld, fl[i]->plan.n, a.N,
+                                                           inverse, var[i]),
+                                     "azimuth fft");
+                }
+            }
+        }
+        return r;
+    };
+    rc = azimuth(0, [&](int i) { return ws_of(i); }, [&](int i) { return (int64_t)d[i].range.n_samples; });
+    if (rc) return rc;
+    // RCMC + matched filter: workspace -> out, grouped by variant
+    {
+        std::vector<int> var(n);
+        for (int i = 0; i < n && !rc; ++i)
+            rc = rcmc_setup(c, d[i].azimuth, prec, cols_of(i), ws_of(i), d[i].range.n_samples, d[i].out, d[i].ld_out,
+                            &var[i]);
+        if (rc) return rc;
+        StageTimer tm(c, S1G_STAGE_RCMC_MF);
+        std::vector<bool> done(n, false);
+        for (int i = 0; i < n && !rc; ++i) {
+            if (done[i]) continue;
+            std::vector<int> group;
+            for (int k = i; k < n; ++k)
+                if (!done[k] && var[k] == var[i]) {
+                    group.push_back(k);
+                    done[k] = true;
+                }
+            if (rcmc_batchable(prec, var[i])) {
+                std::vector<RcmcInst> inst;
+                for (int k : group) {
+                    const s1g_azimuth_desc& a = d[k].azimuth;
+                    RcmcInst r;
+                    r.in = ws_of(k);
+                    r.out = d[k].out;
+                    r.ld_in = a.n_samples;
+                    r.ld_out = d[k].ld_out;
+                    r.cols = cols_of(k);
+                    r.uni = az_uniform_of(cols_of(k), a.n_samples);
+                    r.amp32 = az_amp_of(cols_of(k), a.n_samples);
+                    r.prf = a.prf;
+                    r.n_lines = a.n_lines;
+                    r.N = a.n_samples;
+                    r.apply_rcmc = a.apply_rcmc;
+                    r.nbx = 0;
+                    r.wg0 = 0;
+                    inst.push_back(r);
+                }
+                rc = check_launch(c, launch_rcmc_batch(c->stream, inst.data(), (int)inst.size(), var[i]),
+                                  "rcmc/matched filter");
+            } else {
+                for (int k : group) {
+                    if (rc) break;
+                    rc = check_launch(c, launch_rcmc_mf(c->stream, prec, d[k].azimuth, cols_of(k), ws_of(k),
+                                                        d[k].range.n_samples, d[k].out, d[k].ld_out, var[i]),
+                                      "rcmc/matched filter");
+                }
+            }
+        }
+        if (rc) return rc;
+    }
+    return azimuth(1, [&](int i) { return d[i].out; }, [&](int i) { return d[i].ld_out; });
+}
+
 int s1g_focus_burst(s1g_ctx* c, const s1g_burst_desc* b, int prec) {
     if (!c || !b) return S1G_EARG;
     if (!valid_prec(prec)) return fail(c, S1G_EARG, "precision must be S1G_F32 or S1G_F64%s");
@@ -1256,14 +1460,24 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
         int rc = prepare_burst(c, descs[i], prec, &ws_need, &cols_need);
         if (rc) return rc;
     }
+    const bool staged = (flags & S1G_BATCH_STAGED) && batch_can_overlap(descs, n, prec);
+    if (staged) {
+        std::vector<size_t> wso, colo;
+        size_t wt, ct;
+        staged_layout(descs, n, prec, wso, colo, &wt, &ct);
+        ws_need = std::max(ws_need, wt);
+        cols_need = std::max(cols_need, ct);
+    }
     int rc = ensure(c, &c->ws, &c->ws_bytes, ws_need);
     if (!rc) rc = ensure(c, (void**)&c->cols, &c->cols_n, cols_need);
     if (rc) return rc;
-    if (!(flags & S1G_BATCH_GRAPH)) return batch_chains(c, descs, n, prec);
-    // one hipGraph per distinct descriptor set (and forced variants)
+    auto run_batch = [&]() { return staged ? batch_staged(c, descs, n, prec) : batch_chains(c, descs, n, prec); };
+    if (!(flags & S1G_BATCH_GRAPH)) return run_batch();
+    // one hipGraph per distinct descriptor set (and forced variants and mode)
     std::string key((const char*)descs, sizeof(s1g_burst_desc) * (size_t)n);
     key.append((const char*)&prec, sizeof prec);
     key.append((const char*)c->force, sizeof c->force);
+    key.append(staged ? "S" : "B");
     auto it = c->graphs.find(key);
     if (it != c->graphs.end() && it->second.gen != c->ws_gen) {
         // captured before a workspace reallocation: its pointers are stale
@@ -1276,7 +1490,7 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
     if (it == c->graphs.end()) {
         // one eager pass first: it picks (and caches) each stage's launch
         // variant and builds everything the captured launches use
-        for (int i = 0; i < n && !rc; ++i) rc = burst_chain(c, descs[i], prec);
+        rc = run_batch();
         if (rc) return rc;
         hipStream_t user = c->stream;
         HIPCHK(c, hipStreamSynchronize(user));  // filters built above are complete
@@ -1288,7 +1502,7 @@ int s1g_focus_batch(s1g_ctx* c, const s1g_burst_desc* descs, int n, int prec, in
             c->stream = user;
             return hip_fail(c, e, "hipStreamBeginCapture");
         }
-        rc = batch_chains(c, descs, n, prec);
+        rc = run_batch();
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(c->own, &g);
         c->capturing = false;

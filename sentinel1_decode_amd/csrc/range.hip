@@ -16,6 +16,8 @@
 // geometry and rounded to FP32 for the FP32 chain.
 #include "s1g_internal.h"
 
+#include <algorithm>
+
 namespace s1g {
 
 // g[u] = scale * e^{i theta_u} for u < nc, 0 up to M; FP64 phase with the
@@ -144,11 +146,11 @@ __device__ __forceinline__ int os_pad(int a) { return a + (a >> 4); }
 // one holds outputs n' + S0 n0 and stores them straight to global memory.
 // Twiddles: w_M^e = lo[e mod 128] * hi[e / 128] and w_S0^{r k1} from a
 // [k1][r] table, all in LDS.
+//
+// One launch may serve several bursts (RangeBatch: per-instance lines,
+// filter and buffers; workgroups numbered consecutively across instances).
 template <typename T, int M, int R0, int R1, int R2, int NT>
-__global__ __launch_bounds__(NT) void range_os3_kernel(const cpx<T>* __restrict__ in, int64_t ld_in,
-                                                       cpx<T>* __restrict__ out, int64_t ld_out, int n_lines, int N,
-                                                       int cshift, int V, int nblk, const cpx<T>* __restrict__ tw,
-                                                       const cpx<T>* __restrict__ hs) {
+__global__ __launch_bounds__(NT, (sizeof(T) == 4 ? 2 : 1)) void range_os3_kernel(RangeBatch bt, const cpx<T>* __restrict__ tw) {
     constexpr int S0 = M / R0, S1 = S0 / R1;
     static_assert(S1 == R2 && R0 * R1 * R2 == M, "three passes cover M");
     constexpr int D0 = S0, D1 = M / R1, D2 = M / R2;
@@ -164,11 +166,18 @@ __global__ __launch_bounds__(NT) void range_os3_kernel(const cpx<T>* __restrict_
     cpx<T>* thi = tlo + 128;
     cpx<T>* t1 = thi + NHI;  // [k1][r], R1 * S1 entries
     const int tid = threadIdx.x;
-    const int line = blockIdx.x / nblk;
-    const int blk = blockIdx.x - line * nblk;
-    if (line >= n_lines) return;
+    // the workgroup's instance (uniform: a scan of the kernel arguments)
+    int k = 0;
+    while (k + 1 < bt.n && (int)blockIdx.x >= bt.inst[k + 1].wg0) ++k;
+    const RangeInst& I = bt.inst[k];
+    const int N = I.N, V = I.V, nblk = I.nblk, cshift = I.cshift;
+    const int local = (int)blockIdx.x - I.wg0;
+    const int line = local / nblk;
+    const int blk = local - line * nblk;
+    if (line >= I.n_lines) return;
     const int j0 = blk * V;
-    const cpx<T>* row = in + (int64_t)line * ld_in;
+    const cpx<T>* row = (const cpx<T>*)I.in + (int64_t)line * I.ld_in;
+    const cpx<T>* __restrict__ hs = (const cpx<T>*)I.h;
 
     cpx<T> a[E];
     // pass 0 input: n = tid + S0 n0, in [j0 - cshift, j0 - cshift + M)
@@ -259,7 +268,7 @@ __global__ __launch_bounds__(NT) void range_os3_kernel(const cpx<T>* __restrict_
         a[k0] = cmulc(a[k0], cmul(tlo[e & 127], thi[e >> 7]));
     }
     Dft<T, R0, true>::run(a);
-    cpx<T>* orow = out + (int64_t)line * ld_out;
+    cpx<T>* orow = (cpx<T>*)I.out + (int64_t)line * I.ld_out;
     const int cnt = min(V, N - j0);
 #pragma unroll
     for (int n0 = 0; n0 < R0; ++n0) {
@@ -269,11 +278,53 @@ __global__ __launch_bounds__(NT) void range_os3_kernel(const cpx<T>* __restrict_
 }
 
 template <typename T, int M, int R0, int R1, int R2, int NT>
-static void launch_os3(hipStream_t s, dim3 grid, const void* in, int64_t ld_in, void* out, int64_t ld_out,
-                       int n_lines, int N, int cshift, int V, int nblk, const DevFft& fm, const void* h) {
+static void launch_os3(hipStream_t s, const RangeBatch& bt, int nwg, const DevFft& fm) {
     const size_t lds = sizeof(cpx<T>) * (size_t)(M + M / 16 + 128 + M / 128 + R1 * (M / R0 / R1));
-    hipLaunchKernelGGL((range_os3_kernel<T, M, R0, R1, R2, NT>), grid, dim3(NT), lds, s, (const cpx<T>*)in, ld_in,
-                       (cpx<T>*)out, ld_out, n_lines, N, cshift, V, nblk, tw_of<T>(fm), (const cpx<T>*)h);
+    hipLaunchKernelGGL((range_os3_kernel<T, M, R0, R1, R2, NT>), dim3((unsigned)nwg), dim3(NT), lds, s, bt,
+                       tw_of<T>(fm));
+}
+
+// One instance of a batch (workgroup offset set by the launcher).
+RangeInst range_inst(const void* in, int64_t ld_in, void* out, int64_t ld_out, int n_lines, int n_samples,
+                            int M, int nc, const void* h) {
+    RangeInst r;
+    r.in = in;
+    r.out = out;
+    r.h = h;
+    r.ld_in = ld_in;
+    r.ld_out = ld_out;
+    r.n_lines = n_lines;
+    r.N = n_samples;
+    r.V = M - nc + 1;
+    r.nblk = r.V > 0 ? (n_samples + r.V - 1) / r.V : 0;
+    r.cshift = (nc + 1) / 2;
+    r.wg0 = 0;
+    return r;
+}
+
+bool range_batchable(int prec, int M) { return os_shape(prec, M).m != 0; }
+
+// Several instances whose block length M has a register-pass kernel, in
+// launches of at most kMaxBatch instances.
+int launch_range_os_batch(hipStream_t s, int prec, const DevFft& fm, const RangeInst* inst, int n) {
+    const int M = fm.plan.n;
+    if (!range_batchable(prec, M)) return -1;
+    for (int b0 = 0; b0 < n; b0 += kMaxBatch) {
+        RangeBatch bt;
+        bt.n = std::min(kMaxBatch, n - b0);
+        int nwg = 0;
+        for (int k = 0; k < bt.n; ++k) {
+            bt.inst[k] = inst[b0 + k];
+            if (bt.inst[k].V <= 0) return -1;
+            bt.inst[k].wg0 = nwg;
+            nwg += bt.inst[k].nblk * bt.inst[k].n_lines;
+        }
+        if (nwg == 0) continue;
+        if (prec == S1G_F32 && M == 16384) launch_os3<float, 16384, 32, 32, 16, 512>(s, bt, nwg, fm);
+        else if (prec == S1G_F32) launch_os3<float, 8192, 32, 16, 16, 256>(s, bt, nwg, fm);
+        else launch_os3<double, 8192, 32, 16, 16, 256>(s, bt, nwg, fm);
+    }
+    return (int)hipGetLastError();
 }
 
 int launch_range_os(hipStream_t s, int prec, const DevFft& fm, const void* in, int64_t ld_in, void* out,
@@ -284,18 +335,9 @@ int launch_range_os(hipStream_t s, int prec, const DevFft& fm, const void* in, i
     const int nblk = (n_samples + V - 1) / V;
     const int cshift = (nc + 1) / 2;
     dim3 grid((unsigned)(nblk * n_lines));
-    const OsShape sh = os_shape(prec, M);
-    if (sh.m) {
-        if (prec == S1G_F32 && M == 16384)
-            launch_os3<float, 16384, 32, 32, 16, 512>(s, grid, in, ld_in, out, ld_out, n_lines, n_samples, cshift, V,
-                                                       nblk, fm, h);
-        else if (prec == S1G_F32)
-            launch_os3<float, 8192, 32, 16, 16, 256>(s, grid, in, ld_in, out, ld_out, n_lines, n_samples, cshift, V,
-                                                      nblk, fm, h);
-        else
-            launch_os3<double, 8192, 32, 16, 16, 256>(s, grid, in, ld_in, out, ld_out, n_lines, n_samples, cshift, V,
-                                                       nblk, fm, h);
-        return (int)hipGetLastError();
+    if (range_batchable(prec, M)) {
+        const RangeInst r = range_inst(in, ld_in, out, ld_out, n_lines, n_samples, M, nc, h);
+        return launch_range_os_batch(s, prec, fm, &r, 1);
     }
     constexpr int NT = 512;
     if (prec == S1G_F32) {

@@ -6,6 +6,8 @@
 // per-column table those kernels read.
 #include "s1g_internal.h"
 
+#include <algorithm>
+
 namespace s1g {
 
 // ----------------------------------------------------------- column table
@@ -540,25 +542,36 @@ constexpr int kLinSlack = 64;
 // row tile once.  Other runs, and tiles that would exceed kLinSlack, take
 // exact per-sample weights.  Without linear geometry every sample takes the
 // column table's geometry and exact weights.
+//
+// One launch may serve several bursts (RcmcBatch: per-instance buffers,
+// column table and shape; workgroups numbered consecutively across
+// instances, nbx per Doppler row).
 template <int NT, int EPT>
-__global__ __launch_bounds__(NT) void rcmc32_lin_kernel(s1g_azimuth_desc d, const AzColumn* __restrict__ cols,
-                                                        const AzUniform* __restrict__ uni,
-                                                        const float* __restrict__ amp32,
-                                                        const cpx<float>* __restrict__ in, int64_t ld_in,
-                                                        cpx<float>* __restrict__ out, int64_t ld_out) {
+__global__ __launch_bounds__(NT) void rcmc32_lin_kernel(RcmcBatch bt) {
     constexpr int PER = NT * EPT;
     constexpr int CAP = PER + 2 * kTaps + kLinSlack;
     constexpr int NW = 2 * kTaps + EPT;
     __shared__ cpx<float> tile[CAP];
-    const int N = d.n_samples, L = d.n_lines;
-    const int i = blockIdx.y;
+    // the workgroup's instance (uniform: a scan of the kernel arguments)
+    int kI = 0;
+    while (kI + 1 < bt.n && (int)blockIdx.x >= bt.inst[kI + 1].wg0) ++kI;
+    const RcmcInst& I = bt.inst[kI];
+    const AzColumn* __restrict__ cols = I.cols;
+    const AzUniform* __restrict__ uni = I.uni;
+    const float* __restrict__ amp32 = I.amp32;
+    const cpx<float>* __restrict__ in = (const cpx<float>*)I.in;
+    cpx<float>* __restrict__ out = (cpx<float>*)I.out;
+    const int64_t ld_in = I.ld_in, ld_out = I.ld_out;
+    const int N = I.N, L = I.n_lines;
+    const int local = (int)blockIdx.x - I.wg0;
+    const int i = local / I.nbx;
     const int tid = threadIdx.x;
-    const int j0 = blockIdx.x * PER;
-    const bool rcmc = d.apply_rcmc != 0;
+    const int j0 = (local - i * I.nbx) * PER;
+    const bool rcmc = I.apply_rcmc != 0;
     double fa;
     {
 #pragma clang fp contract(off)
-        const double start = -d.prf / 2, end = d.prf / 2;
+        const double start = -I.prf / 2, end = I.prf / 2;
         const double step = (end - start) / (L - 1);
         fa = start + i * step;
     }
@@ -675,14 +688,56 @@ __global__ __launch_bounds__(NT) void rcmc32_lin_kernel(s1g_azimuth_desc d, cons
 }
 
 template <int NT, int EPT>
+static void launch_rcmc32_lin(hipStream_t s, const RcmcInst* inst, int n) {
+    constexpr int PER = NT * EPT;
+    for (int b0 = 0; b0 < n; b0 += kMaxBatch) {
+        RcmcBatch bt;
+        bt.n = std::min(kMaxBatch, n - b0);
+        int nwg = 0;
+        for (int k = 0; k < bt.n; ++k) {
+            bt.inst[k] = inst[b0 + k];
+            bt.inst[k].nbx = (bt.inst[k].N + PER - 1) / PER;
+            bt.inst[k].wg0 = nwg;
+            nwg += bt.inst[k].nbx * bt.inst[k].n_lines;
+        }
+        if (nwg > 0) hipLaunchKernelGGL((rcmc32_lin_kernel<NT, EPT>), dim3((unsigned)nwg), dim3(NT), 0, s, bt);
+    }
+}
+
+static RcmcInst rcmc_inst(const s1g_azimuth_desc& d, const AzColumn* cols, const void* in, int64_t ld_in, void* out,
+                          int64_t ld_out) {
+    RcmcInst r;
+    AzColumn* c = const_cast<AzColumn*>(cols);
+    r.in = in;
+    r.out = out;
+    r.ld_in = ld_in;
+    r.ld_out = ld_out;
+    r.cols = cols;
+    r.uni = az_uniform_of(c, d.n_samples);
+    r.amp32 = az_amp_of(c, d.n_samples);
+    r.prf = d.prf;
+    r.n_lines = d.n_lines;
+    r.N = d.n_samples;
+    r.apply_rcmc = d.apply_rcmc;
+    r.nbx = 0;
+    r.wg0 = 0;
+    return r;
+}
+
+template <int NT, int EPT>
 static void launch_rcmc32_lin(hipStream_t s, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
                               int64_t ld_in, void* out, int64_t ld_out) {
-    const int per = NT * EPT;
-    dim3 grid((unsigned)((d.n_samples + per - 1) / per), (unsigned)d.n_lines);
-    AzColumn* c = const_cast<AzColumn*>(cols);
-    hipLaunchKernelGGL((rcmc32_lin_kernel<NT, EPT>), grid, dim3(NT), 0, s, d, cols,
-                       (const AzUniform*)az_uniform_of(c, d.n_samples), (const float*)az_amp_of(c, d.n_samples),
-                       (const cpx<float>*)in, ld_in, (cpx<float>*)out, ld_out);
+    const RcmcInst r = rcmc_inst(d, cols, in, ld_in, out, ld_out);
+    launch_rcmc32_lin<NT, EPT>(s, &r, 1);
+}
+
+bool rcmc_batchable(int prec, int variant) { return prec == S1G_F32 && (variant == 5 || variant == 6); }
+
+int launch_rcmc_batch(hipStream_t s, const RcmcInst* inst, int n, int variant) {
+    if (variant == 5) launch_rcmc32_lin<256, 16>(s, inst, n);
+    else if (variant == 6) launch_rcmc32_lin<256, 8>(s, inst, n);
+    else return -1;
+    return (int)hipGetLastError();
 }
 
 int rcmc_variants(int prec) { return prec == S1G_F32 ? 7 : 3; }

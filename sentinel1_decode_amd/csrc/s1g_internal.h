@@ -97,11 +97,63 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
 int launch_fft_global(hipStream_t stream, int prec, const DevFft& f, const void* in, void* out, void* tmp, int batch,
                       int64_t dist, int inverse);
 
+// Several bursts in one launch: per-instance parameters passed by value in
+// the kernel arguments (a workgroup finds its instance by a scan of them),
+// at most kMaxBatch instances per launch.
+constexpr int kMaxBatch = 16;
+struct RangeInst {
+    const void* in;
+    void* out;
+    const void* h;  // the instance's overlap-save filter
+    int64_t ld_in, ld_out;
+    int n_lines, N, V, nblk, cshift;
+    int wg0;  // first workgroup of the instance (set by the launcher)
+};
+struct RangeBatch {
+    int n;
+    RangeInst inst[kMaxBatch];
+};
+struct AzInst {
+    void* data;
+    int64_t ld;
+    int N;
+    int panel0;  // first panel of the instance (set by the launcher)
+};
+struct AzBatch {
+    int n, npanels;
+    AzInst inst[kMaxBatch];
+};
+struct AzColumn;
+struct AzUniform;
+struct RcmcInst {
+    const void* in;
+    void* out;
+    int64_t ld_in, ld_out;
+    const AzColumn* cols;  // the instance's column table (launch_az_columns)
+    const AzUniform* uni;  // its linear-geometry summary and FP32 amplitudes
+    const float* amp32;
+    double prf;
+    int n_lines, N, apply_rcmc;
+    int nbx;  // workgroups per Doppler row (set by the launcher)
+    int wg0;  // first workgroup of the instance (set by the launcher)
+};
+struct RcmcBatch {
+    int n;
+    RcmcInst inst[kMaxBatch];
+};
+
 // Range compression: build the overlap-save filter, then filter the lines.
 int launch_range_filter(hipStream_t stream, const DevFft& fm, double txpsf, double txprr, double txpl, double fs_mhz,
                         int n_samples, int nc, void* scratch, void* h_out, int prec);
 int launch_range_os(hipStream_t stream, int prec, const DevFft& fm, const void* in, int64_t ld_in, void* out,
                     int64_t ld_out, int n_lines, int n_samples, int nc, const void* h);
+// Batched form: instances sharing the block length fm.plan.n, which must
+// have a register-pass kernel (range_batchable).  range_inst fills one
+// instance.
+bool range_batchable(int prec, int block_len);
+RangeInst range_inst(const void* in, int64_t ld_in, void* out, int64_t ld_out, int n_lines, int n_samples,
+                     int block_len, int nc, const void* h);
+int launch_range_os_batch(hipStream_t stream, int prec, const DevFft& fm, const RangeInst* inst, int n);
 
 // Azimuth stages.
 // Launch variants (candidates of the autotuner): variant counts and the
@@ -112,7 +164,16 @@ int rcmc_variants(int prec);
 // shift = 0: plain transform.  The inverse is scaled by 1/n_lines.
 int launch_azimuth_fft(hipStream_t stream, int prec, const DevFft& fl, void* data, int64_t ld, int n_lines,
                        int n_samples, int inverse, int variant, int shift = 1);
+// Batched form (instances of one azimuth length): the persistent variants
+// (azimuth_batchable) walk the panels of every instance in one launch.
+bool azimuth_batchable(const DevFft& fl, int variant);
+int launch_azimuth_fft_batch(hipStream_t stream, int prec, const DevFft& fl, const AzInst* inst, int n, int inverse,
+                             int variant);
 int launch_az_columns(hipStream_t stream, const s1g_azimuth_desc& d, AzColumn* cols);
+// Batched RCMC + matched filter for the FP32 linear-geometry variants
+// (rcmc_batchable); each instance's column table filled beforehand.
+bool rcmc_batchable(int prec, int variant);
+int launch_rcmc_batch(hipStream_t stream, const RcmcInst* inst, int n, int variant);
 int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, const AzColumn* cols, const void* in,
                    int64_t ld_in, void* out, int64_t ld_out, int variant);
 

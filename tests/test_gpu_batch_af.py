@@ -92,6 +92,49 @@ def test_batch_with_eccm_matches_single_bursts(ctx, uneven_bursts, graph):
         assert torch.equal(flags, fa), rep
 
 
+@pytest.fixture(scope="module")
+def iw_class_bursts():
+    """Bursts of the IW azimuth lengths (compiled radix sequences): two of
+    one length (one launch covers both), one each of the others."""
+    specs = [synth.BurstSpec(n_lines=1480, n_samples=2400, rgdec=8, pri_code=21861, n_targets=20,
+                             amp_mode="rayleigh", f_dc=30.0, seed=41),
+             synth.BurstSpec(n_lines=1420, n_samples=2160, rgdec=6, pri_code=22276, n_targets=20,
+                             amp_mode="rayleigh", seed=42),
+             synth.BurstSpec(n_lines=1480, n_samples=2000, rgdec=8, pri_code=21861, n_targets=20,
+                             amp_mode="rayleigh", seed=43),
+             synth.BurstSpec(n_lines=1520, n_samples=2040, rgdec=11, pri_code=25868, n_targets=20,
+                             amp_mode="rayleigh", seed=44)]
+    return [pipeline.from_synth(synth.make_burst(s)) for s in specs]
+
+
+@pytest.mark.parametrize("prec", [F32, F64])
+@pytest.mark.parametrize("az_variant", [-1, 3, 4])
+def test_batch_stages_match_burstwise(ctx, iw_class_bursts, prec, az_variant):
+    """The stage-ordered batch (S1G_BATCH_STAGED: one launch per stage and
+    shape class, the persistent kernels driven by per-burst descriptors)
+    gives every burst the bits of its own chain, eager and as a replayed
+    graph, as does the default burst-by-burst batch, with the azimuth
+    variant chosen by the tuner or forced to either persistent form."""
+    dt = torch.complex64 if prec == F32 else torch.complex128
+    for st in ("az_fft", "az_ifft"):
+        ctx.force_variant(st, az_variant)
+    try:
+        single = [db.focus(ctx, prec) for db in iw_class_bursts]
+        torch.cuda.synchronize()
+        outs = [torch.zeros((db.n_lines, db.n_samples), dtype=dt, device="cuda") for db in iw_class_bursts]
+        descs = [db.burst_desc(o) for db, o in zip(iw_class_bursts, outs)]
+        for staged, graph in ((True, False), (True, True), (True, True), (False, False)):
+            for o in outs:
+                o.zero_()
+            ctx.focus_batch(descs, prec, graph=graph, staged=staged)
+            torch.cuda.synchronize()
+            for k, (o, e) in enumerate(zip(outs, single)):
+                assert torch.equal(o, e), (staged, graph, k)
+    finally:
+        for st in ("az_fft", "az_ifft"):
+            ctx.force_variant(st, -1)
+
+
 def _af_problem(L=256, N=96, seed=3):
     rng = np.random.default_rng(seed)
     rd = (rng.standard_normal((L, N)) + 1j * rng.standard_normal((L, N)))
