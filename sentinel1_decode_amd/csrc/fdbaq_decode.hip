@@ -1,4 +1,4 @@
-// fdbaq_decode.hip -- Level-0 FDBAQ (data format D) decode, one wavefront per
+// fdbaq_decode.hip -- Level-0 FDBAQ (data format D) decode, one workgroup per
 // space packet.
 //
 // Reference behaviour (AndrewPlayer3/sentinel1_decode): L0Packet::_decode
@@ -417,11 +417,14 @@ __device__ __forceinline__ int region_prefix(const Region& r, int delta, int lan
 //     that outruns the region goes on from the region's end.
 template <int BRC, bool ALL>
 __device__ int resolve_block(const PacketWords<ALL>& words, int nbits, Region& r, int pre, int pred, int start,
-                             int nsym, int delta, int lane, bool want_hdr, uint32_t* hdr) {
+                             int nsym, int delta, uint32_t dmagic, int lane, bool want_hdr, uint32_t* hdr) {
     const int off = start - pred;
     if (start > nbits || off < 0 || off >= 64 * delta)
         return walk_block<BRC, ALL>(words, nbits, start, nsym, delta, lane, want_hdr, hdr);
-    const int k = off / delta, e = off - k * delta;
+    // off / delta as a multiply-high by dmagic = floor((2^32 - 1) / delta) + 1,
+    // exact for off < 64 delta and delta <= 32 (the division is on the
+    // block's critical path; dmagic was formed while the region was walked)
+    const int k = (int)__umulhi((uint32_t)off, dmagic), e = off - k * delta;
     const uint32_t mk = (uint32_t)__builtin_amdgcn_readlane((int)r.mask, k);
     const int ek = __builtin_amdgcn_readlane(r.entry, k);
     if (ek <= e && ((mk >> e) & 1u)) {
@@ -523,9 +526,15 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
 }
 
 // Bits the speculative region of a block starts before the predicted start
-// (the previous block's start plus IE's length of that block): channels of
-// one block share its BRC and statistics, so the lengths differ by ~16 bits.
+// (a known block start plus IE's lengths of the blocks in between): channels
+// of one block share its BRC and statistics, so the lengths differ by ~16
+// bits per block; the margin grows with the number of blocks predicted over.
 constexpr int kSpecMargin = 40;
+constexpr int kSpecMarginStep = 24;
+// Waves that walk the IO, QE and QO blocks, block b on wave b mod kWalkers;
+// one more wave decodes values.
+constexpr int kWalkers = 3;
+constexpr int kDecodeThreads = 64 * (kWalkers + 1);
 // Progress value of a finished (or abandoned) channel.
 constexpr int kDone = 1 << 20;
 
@@ -555,33 +564,53 @@ __device__ __forceinline__ bool wait_for(int* flag, int need, int* err, int lane
     return false;
 }
 
-// Three wavefronts per packet.
+// Waits (wave-uniform, no sleep: it is on the walk's critical path) until
+// the in-channel progress word reaches `need`, with one LDS load per poll.
+// Progress within a channel never reaches kDone before block need - 1 is
+// published, so kDone means an error was posted: false.  The bound only
+// guards against a broken invariant (it posts S1G_PKT_ERR_ARG).
+__device__ __forceinline__ bool wait_start(int* flag, int need, int* err, int lane) {
+    for (int spin = 0; spin < (1 << 26); ++spin) {
+        const int v = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (v >= need) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the block start written before v
+            return v < kDone;
+        }
+    }
+    if (lane == 0) __hip_atomic_store(err, S1G_PKT_ERR_ARG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return false;
+}
+
+// kWalkers + 1 wavefronts per packet.
 //   wave 0 walks IE block by block (each block's BRC is in the stream);
-//   waves 0 and 1 then walk IO, QE and QO as a pipeline: block b is walked
-//     by wave b & 1.  Its BRC is known from IE, so before its start is known
-//     the wave speculates: it walks and settles a region from the previous
-//     block's start plus IE's length of that block, less kSpecMargin bits.
-//     When the other wave publishes the start (an LDS word with release /
-//     acquire at workgroup scope), the chain is settled again from the lane
-//     that holds it and counted -- usually one round -- so each block costs
-//     about half a walk on the critical path;
-//   wave 2 decodes values, in groups of 16 blocks (lane 4j + ch takes
+//   waves 0 .. kWalkers - 1 then walk IO, QE and QO as a pipeline: block b
+//     is walked by wave b mod kWalkers.  Its BRC is known from IE, so before
+//     its start is known the wave speculates: it walks and settles a region
+//     from the latest block start it knows (the block after the one it last
+//     resolved) plus IE's lengths of the blocks in between, less a margin
+//     that grows with their number.  When the wave before it publishes the
+//     start (an LDS word with release / acquire at workgroup scope), the
+//     chain from there is the settled chain whenever the start is one of its
+//     recorded symbol starts -- one ballot -- so each block costs about
+//     1/kWalkers of a walk on the critical path;
+//   wave kWalkers decodes values, in groups of 16 blocks (lane 4j + ch takes
 //     channel ch of the group's block j, so each store instruction writes
-//     whole samples), as soon as the group's QO starts are published; waves
-//     0 and 1 join it when the walks end.
+//     whole samples), as soon as the group's QO starts are published; the
+//     walking waves join it when the walks end.
 // Errors are found in block order (a block is resolved only after the one
 // before it), posted once, and end every wait.  The walks are the packet's
 // critical path, so the walking waves raise their issue priority
 // (s_setprio) over the value decodes sharing their SIMDs.
 template <typename T, bool ALL>
-__global__ __launch_bounds__(192) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
+__global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
                                                            const uint64_t* __restrict__ offs,
                                                            const uint32_t* __restrict__ lens,
                                                            const int32_t* __restrict__ nqs, int n_packets,
                                                            int max_words, int lds_words, int max_blocks,
                                                            int max_quads, T* __restrict__ out, int64_t ld_out,
                                                            int32_t* __restrict__ status) {
-    constexpr int kThreads = 192;
+    constexpr int kThreads = kDecodeThreads;
     extern __shared__ __align__(16) uint32_t smem[];  // base 16-aligned after the static words
     __shared__ int s_err;          // first error status
     __shared__ int s_pub[4];       // blocks of each channel whose start is published; kDone when finished
@@ -727,8 +756,8 @@ __global__ __launch_bounds__(192) void fdbaq_decode_kernel(const uint8_t* __rest
         __builtin_amdgcn_s_setprio(0);
     }
 
-    if (wave < 2) {
-        // ---- IO, QE, QO: blocks alternate between waves 0 and 1
+    if (wave < kWalkers) {
+        // ---- IO, QE, QO: block b on wave b mod kWalkers
         __builtin_amdgcn_s_setprio(2);
         for (int ch = 1; ch < 4; ++ch) {
             if (!wait_for<4>(&s_pub[ch - 1], kDone, &s_err, lane)) break;
@@ -759,9 +788,10 @@ __global__ __launch_bounds__(192) void fdbaq_decode_kernel(const uint8_t* __rest
                 bstart[ch * max_blocks] = s0;
                 lds_store_rel(&s_pub[ch], 1);
             }
-            int base = s0;  // start of the block before this wave's next one
+            // the latest block start this wave knows: block kb starts at base
+            int base = s0, kb = 0;
             bool stop = false;
-            for (int b = wave; b < nb && !stop; b += 2) {
+            for (int b = wave; b < nb && !stop; b += kWalkers) {
                 const int brc = __builtin_amdgcn_readfirstlane((int)bbrc[b]);
                 const int nsym = (b == nb - 1) ? nq - 128 * (nb - 1) : 128;
                 uint32_t hdr = 0;
@@ -770,10 +800,16 @@ __global__ __launch_bounds__(192) void fdbaq_decode_kernel(const uint8_t* __rest
                     const int delta = min(max((ie_len(0) + 63) / 64 + 1, 2), 32);
                     end = walk_any(brc, s0, nsym, delta, ch == 2, &hdr);
                 } else {
-                    // clamped to the data: a region from at most nbits reads
-                    // no further than the staged zero words reach
-                    const int pred = min(max(base + ie_len(b - 1) + hbits - kSpecMargin, base), nbits);
-                    const int delta = min(max((ie_len(b) + 2 * kSpecMargin + 16 + 63) / 64 + 1, 2), 32);
+                    // predicted from block kb's start over IE's lengths of
+                    // blocks kb .. b - 1; clamped to the data: a region from
+                    // at most nbits reads no further than the staged zero
+                    // words reach
+                    int est = base;
+                    for (int q = kb; q < b; ++q) est += ie_len(q) + hbits;
+                    const int margin = kSpecMargin + kSpecMarginStep * (b - kb - 1);
+                    const int pred = min(max(est - margin, base), nbits);
+                    const int delta = min(max((ie_len(b) + 2 * margin + 16 + 63) / 64 + 1, 2), 32);
+                    const uint32_t dmagic = 0xFFFFFFFFu / (uint32_t)delta + 1u;
                     Region r;
                     int pre;
                     switch (brc) {
@@ -784,14 +820,14 @@ __global__ __launch_bounds__(192) void fdbaq_decode_kernel(const uint8_t* __rest
                         default: region_walk<4, ALL>(words, pred, delta, lane, r); region_settle<4>(r, 0, 0, delta, lane); break;
                     }
                     pre = region_prefix(r, delta, lane);
-                    if (!wait_for<1>(&s_pub[ch], b + 1, &s_err, lane)) break;
+                    if (!wait_start(&s_pub[ch], b + 1, &s_err, lane)) break;
                     const int start = __builtin_amdgcn_readfirstlane(bstart[ch * max_blocks + b]);
                     switch (brc) {
-                        case 0: end = resolve_block<0, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
-                        case 1: end = resolve_block<1, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
-                        case 2: end = resolve_block<2, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
-                        case 3: end = resolve_block<3, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
-                        default: end = resolve_block<4, ALL>(words, nbits, r, pre, pred, start, nsym, delta, lane, ch == 2, &hdr); break;
+                        case 0: end = resolve_block<0, ALL>(words, nbits, r, pre, pred, start, nsym, delta, dmagic, lane, ch == 2, &hdr); break;
+                        case 1: end = resolve_block<1, ALL>(words, nbits, r, pre, pred, start, nsym, delta, dmagic, lane, ch == 2, &hdr); break;
+                        case 2: end = resolve_block<2, ALL>(words, nbits, r, pre, pred, start, nsym, delta, dmagic, lane, ch == 2, &hdr); break;
+                        case 3: end = resolve_block<3, ALL>(words, nbits, r, pre, pred, start, nsym, delta, dmagic, lane, ch == 2, &hdr); break;
+                        default: end = resolve_block<4, ALL>(words, nbits, r, pre, pred, start, nsym, delta, dmagic, lane, ch == 2, &hdr); break;
                     }
                 }
                 if (end > nbits) {
@@ -819,6 +855,7 @@ __global__ __launch_bounds__(192) void fdbaq_decode_kernel(const uint8_t* __rest
                     lds_store_rel(&s_pub[ch], kDone);
                 }
                 base = next;
+                kb = b + 1;
             }
             if (stop) break;
         }
@@ -864,7 +901,7 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     const int nrl_words = (lds_words + 4 * max_blocks + (2 * max_blocks + 3) / 4 + 1) & ~1;
     const size_t lds = sizeof(uint32_t) * (size_t)nrl_words + sizeof(double) * 80;
     if (lds > 160 * 1024) return -1;
-    dim3 grid(n_packets), block(192);
+    dim3 grid(n_packets), block(kDecodeThreads);
     const bool all = max_words + kPadWords <= kLdsWords;
 #define S1G_LAUNCH(T, A)                                                                                       \
     hipLaunchKernelGGL((fdbaq_decode_kernel<T, A>), grid, block, lds, stream, ud, offs, lens, nqs, n_packets, \
