@@ -474,26 +474,16 @@ int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_
     if (fsz < 4 || fsz > 1024 || (fsz & (fsz - 1)) || d.stride < 1 || d.stride > fsz)
         return fail(c, S1G_EUNSUPPORTED, "ECCM window must be a power of two in [4, 1024] with 1 <= stride <= size%s");
     if ((fsz + d.stride - 1) / d.stride > 8) return fail(c, S1G_EUNSUPPORTED, "ECCM window overlap too deep%s");
-    // detection split r: N/r must fit 128 KB of LDS and factor over the
-    // radices (a 64 KB bound, r = 3 instead of 2 for the IW2 FP32 line, gave
-    // two workgroups per CU but 0.70 against 0.61 ms: each reads the line)
-    int r = 0;
-    static const int cand[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32};
-    for (int k : cand) {
-        if (N % k) continue;
-        FftPlan p;
-        if ((size_t)(N / k) * esize(prec) <= 128 * 1024 && factorise(N / k, p)) {
-            r = k;
-            break;
-        }
-    }
+    // detection split r: each sub-spectrum N/r is transformed in LDS (a
+    // 128 KB bound, r = 2 for the IW2 FP32 line, ran one workgroup per CU)
+    const int r = eccm_detect_split(prec, N, factorise);
     if (!r) return fail(c, S1G_EUNSUPPORTED, "ECCM detection length does not split over the LDS radices%s");
     const DevFft *fN, *fsub, *fw;
     int rc = get_fft(c, N, &fN);
     if (!rc) rc = get_plan(c, N / r, &fsub);
     if (!rc) rc = get_plan(c, fsz, &fw);
     if (rc) return rc;
-    rc = ensure(c, &c->eccm_scratch, &c->eccm_scratch_bytes, sizeof(double) * (size_t)L * r);
+    rc = ensure(c, &c->eccm_scratch, &c->eccm_scratch_bytes, sizeof(double) * (size_t)L);
     if (rc) return rc;
     StageTimer tm(c, S1G_STAGE_ECCM);
     return check_launch(c, launch_eccm(c->stream, prec, d, *fN, *fsub, r, *fw, in, ld_in, out, ld_out, flags,
@@ -929,9 +919,10 @@ int s1g_eccm(s1g_ctx* c, const s1g_eccm_desc* d, int prec, void* data, int64_t l
 }
 
 // ------------------------------------------------------------ general DSP
-// Rows x n DFT along rows, in place: Stockham passes when n factors over
-// the pass radices, else Bluestein through a power-of-two length.  Rows go
-// in batches that keep the work arrays within kDftWorkBudget bytes.
+// Rows x n DFT along rows, in place: when n factors over the pass radices,
+// one workgroup per row in LDS if the row fits, else global Stockham passes;
+// otherwise Bluestein through a power-of-two length.  The global forms go in
+// batches of rows that keep the work arrays within kDftWorkBudget bytes.
 constexpr size_t kDftWorkBudget = size_t(4) << 30;
 
 static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, int inverse, bool normalise = true) {
@@ -941,6 +932,11 @@ static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t l
     if (rc) return rc;
     const size_t es = esize(prec);
     const double scale = inverse && normalise ? 1.0 / (double)n : 1.0;
+    if (f->plan.npass >= 0 && fft_rows_lds_fits(prec, n)) {
+        // whole rows in LDS: one read and one write per row
+        rc = launch_fft_rows_lds(c->stream, prec, *f, data, ld, rows, inverse, scale);
+        return check_launch(c, rc, "dft");
+    }
     if (f->plan.npass >= 0) {
         const int chunk = (int)std::max<size_t>(1, std::min<size_t>(rows, kDftWorkBudget / (2 * es * (size_t)ld)));
         rc = ensure(c, &c->dft_ws, &c->dft_ws_bytes, 2 * es * (size_t)chunk * ld);
@@ -1436,14 +1432,10 @@ static int prepare_burst(s1g_ctx* c, const s1g_burst_desc& b, int prec, size_t* 
         const DevFft* f;
         rc = get_fft(c, N, &f);
         if (!rc) rc = get_plan(c, b.eccm.fft_size, &f);
-        for (int k : {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32}) {
-            if (rc || N % k) continue;
-            FftPlan p;
-            if ((size_t)(N / k) * esize(prec) <= 128 * 1024 && factorise(N / k, p)) {
-                rc = get_plan(c, N / k, &f);
-                if (!rc) rc = ensure(c, &c->eccm_scratch, &c->eccm_scratch_bytes, sizeof(double) * (size_t)L * k);
-                break;
-            }
+        const int k = eccm_detect_split(prec, N, factorise);
+        if (!rc && k) {
+            rc = get_plan(c, N / k, &f);
+            if (!rc) rc = ensure(c, &c->eccm_scratch, &c->eccm_scratch_bytes, sizeof(double) * (size_t)L);
         }
     }
     *ws_need = std::max(*ws_need, esize(prec) * (size_t)L * N);

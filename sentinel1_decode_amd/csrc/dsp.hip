@@ -10,6 +10,7 @@
 // (image_formation.cpp:4-43), L0Packet::get_replica_chirp
 // (packet.cpp:345-373), sinc_interpolate (signal_processing.cpp:228-257).
 #include "s1g_internal.h"
+#include "s1g_lds_fft.h"
 
 namespace s1g {
 
@@ -34,6 +35,90 @@ int launch_copy_scale(hipStream_t s, int prec, const void* src, int64_t lds, voi
     else
         hipLaunchKernelGGL(copy_scale_kernel<double>, g, b, 0, s, (const cpx<double>*)src, lds, (cpx<double>*)dst,
                            ldd, rows, n, scale);
+    return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------- LDS row transforms
+// One workgroup per row, in place: the row is loaded into LDS at its
+// digit-reversed positions, the plan's DIT passes run there with the
+// two-level twiddle tables in LDS, and the store writes the natural-order
+// spectrum with the scale folded in.  One read and one write of the row,
+// against one per pass plus a copy for the global Stockham passes.
+template <typename T, bool INV, int NT, int MAXR, int MINW>
+__global__ __launch_bounds__(NT, MINW) void fft_rows_lds_kernel(cpx<T>* __restrict__ data, int64_t ld, int rows,
+                                                                FftPlan plan, const cpx<T>* __restrict__ tw,
+                                                                const int32_t* __restrict__ perm, T scale) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int n = plan.n, tid = threadIdx.x;
+    cpx<T>* buf = (cpx<T>*)smem_raw;
+    cpx<T>* lo = buf + n;
+    cpx<T>* hi = lo + kTwLo;
+    const int row = blockIdx.x;
+    if (row >= rows) return;
+    cpx<T>* x = data + (int64_t)row * ld;
+    load_tw2(tw, n, lo, hi, tid, NT);
+    for (int q = tid; q < n; q += NT) buf[perm[q]] = x[q];
+    __syncthreads();
+    fft_dit_lds_tw2<T, INV, MAXR>(buf, plan, lo, hi, tid, NT);
+    for (int q = tid; q < n; q += NT) x[q] = scl(buf[q], scale);
+}
+
+static size_t rows_lds_bytes(int prec, int n) {
+    return (size_t)(n + kTwLo + tw_hi_count(n)) * (prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>));
+}
+
+bool fft_rows_lds_fits(int prec, int n) { return n > 0 && rows_lds_bytes(prec, n) <= kRowLdsBytes; }
+
+// Minimum waves per SIMD asked of the compiler per radix class: the
+// register budget of a class's largest butterfly (FP64 needs twice the
+// registers of FP32).  Rows above half the LDS leave one workgroup per CU,
+// which then runs 512 threads (two waves per SIMD: up to 256 registers).
+template <typename T, int CLS, int NT>
+constexpr int rows_min_waves() {
+    if constexpr (NT == 512) return 2;
+    else if constexpr (sizeof(T) == 4) return CLS <= 1 ? 4 : 2;
+    else return CLS == 0 ? 4 : CLS == 1 ? 2 : 1;
+}
+
+template <typename T, int CLS, int NT>
+static void launch_rows_cls(hipStream_t s, const DevFft& f, void* data, int64_t ld, int rows, int inverse,
+                            double scale, size_t lds) {
+    constexpr int MAXR = kRadixClassMax[CLS], MINW = rows_min_waves<T, CLS, NT>();
+    const dim3 g((unsigned)rows);
+    if (inverse)
+        hipLaunchKernelGGL((fft_rows_lds_kernel<T, true, NT, MAXR, MINW>), g, dim3(NT), lds, s, (cpx<T>*)data, ld,
+                           rows, f.plan, tw_of<T>(f), f.perm, (T)scale);
+    else
+        hipLaunchKernelGGL((fft_rows_lds_kernel<T, false, NT, MAXR, MINW>), g, dim3(NT), lds, s, (cpx<T>*)data, ld,
+                           rows, f.plan, tw_of<T>(f), f.perm, (T)scale);
+}
+
+template <typename T, int NT>
+static void launch_rows_nt(hipStream_t s, const DevFft& f, void* data, int64_t ld, int rows, int inverse,
+                           double scale, size_t lds) {
+    switch (radix_class(f.plan)) {
+        case 0: launch_rows_cls<T, 0, NT>(s, f, data, ld, rows, inverse, scale, lds); break;
+        case 1: launch_rows_cls<T, 1, NT>(s, f, data, ld, rows, inverse, scale, lds); break;
+        case 2: launch_rows_cls<T, 2, NT>(s, f, data, ld, rows, inverse, scale, lds); break;
+        default: launch_rows_cls<T, 3, NT>(s, f, data, ld, rows, inverse, scale, lds); break;
+    }
+}
+
+template <typename T>
+static void launch_rows_t(hipStream_t s, const DevFft& f, void* data, int64_t ld, int rows, int inverse,
+                          double scale, size_t lds) {
+    if (lds > kRowLdsBytes / 2) launch_rows_nt<T, 512>(s, f, data, ld, rows, inverse, scale, lds);
+    else launch_rows_nt<T, 256>(s, f, data, ld, rows, inverse, scale, lds);
+}
+
+int launch_fft_rows_lds(hipStream_t s, int prec, const DevFft& f, void* data, int64_t ld, int rows, int inverse,
+                        double scale) {
+    const int n = f.plan.n;
+    if (rows <= 0) return 0;
+    if (f.plan.npass < 0 || !fft_rows_lds_fits(prec, n)) return -1;
+    const size_t lds = rows_lds_bytes(prec, n);
+    if (prec == S1G_F32) launch_rows_t<float>(s, f, data, ld, rows, inverse, scale, lds);
+    else launch_rows_t<double>(s, f, data, ld, rows, inverse, scale, lds);
     return (int)hipGetLastError();
 }
 

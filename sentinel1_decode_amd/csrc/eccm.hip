@@ -14,48 +14,62 @@
 //
 // Detection: the N-point spectrum maximum is found as the maximum over r
 // interleaved sub-spectra X[r k + q] = FFT_{N/r}(y_q)[k] with
-// y_q[n] = sum_m x[n + m N/r] w_r^{q m} w_N^{q n}, each of which fits LDS.
+// y_q[n] = sum_m x[n + m N/r] w_r^{q m} w_N^{q n}.  One workgroup per line
+// forms and transforms the r sub-spectra in turn (the line comes from HBM
+// once; the later sub-spectra re-read it from cache), each in LDS with the
+// sub-length's twiddles in LDS too; the maximum does not depend on the
+// order of the bins, so natural-order input and DIF passes need no
+// digit-reversal.
 // Notch: one workgroup per (line, run of windows); unflagged lines are copied.
 #include "s1g_internal.h"
+#include "s1g_lds_fft.h"
 
 namespace s1g {
 
-template <typename T, int NT>
+template <typename T, int NT, int MAXR>
 __global__ __launch_bounds__(NT) void eccm_detect_kernel(const cpx<T>* __restrict__ in, int64_t ld, int N, int r,
                                                          FftPlan sub, const cpx<T>* __restrict__ twN,
-                                                         const cpx<T>* __restrict__ twSub,
-                                                         const int32_t* __restrict__ permSub,
-                                                         T* __restrict__ partial) {
+                                                         const cpx<T>* __restrict__ twSub, T* __restrict__ peak) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cpx<T>* buf = (cpx<T>*)smem_raw;
-    __shared__ T red[NT / 64];
-    const int line = blockIdx.x / r, q = blockIdx.x % r;
-    const int tid = threadIdx.x;
     const int n1 = N / r;
+    cpx<T>* buf = (cpx<T>*)smem_raw;   // n1
+    cpx<T>* lo = buf + n1;             // kTwLo
+    cpx<T>* hi = lo + kTwLo;           // tw_hi_count(n1)
+    cpx<T>* wr = hi + tw_hi_count(n1); // r: w_r^k = w_N^{k N / r}
+    __shared__ T red[NT / 64];
+    const int line = blockIdx.x;
+    const int tid = threadIdx.x;
     const cpx<T>* row = in + (int64_t)line * ld;
-    for (int n = tid; n < n1; n += NT) {
-        cpx<T> y = mk((T)0, (T)0);
-        for (int m = 0; m < r; ++m) {
-            const cpx<T> x = row[n + m * n1];
-            const int k = (int)(((int64_t)q * m % r) * n1);  // w_r^{qm} = w_N^{qm N/r}
-            y = y + cmul(x, twN[k]);
-        }
-        if (q) y = cmul(y, twN[q * n]);
-        buf[permSub[n]] = y;
-    }
+    load_tw2(twSub, n1, lo, hi, tid, NT);
+    for (int k = tid; k < r; k += NT) wr[k] = twN[(int64_t)k * n1];
     __syncthreads();
-    fft_dit_lds<T, false>(buf, sub, 1, 0, 1, twSub, tid, NT);
     T mx = 0;
-    for (int k = tid; k < n1; k += NT) {
-        const cpx<T> v = buf[k];
-        mx = fmax(mx, v.x * v.x + v.y * v.y);
+    for (int q = 0; q < r; ++q) {
+        for (int n = tid; n < n1; n += NT) {
+            cpx<T> y = row[n];
+            int k = 0;  // (q m) mod r
+            for (int m = 1; m < r; ++m) {
+                k += q;
+                if (k >= r) k -= r;
+                y = y + cmul(row[n + m * n1], wr[k]);
+            }
+            if (q) y = cmul(y, twN[q * n]);
+            buf[n] = y;
+        }
+        __syncthreads();
+        fft_dif_lds_tw2<T, false, MAXR>(buf, sub, lo, hi, tid, NT);
+        for (int k = tid; k < n1; k += NT) {
+            const cpx<T> v = buf[k];
+            mx = fmax(mx, v.x * v.x + v.y * v.y);
+        }
+        __syncthreads();  // every lane has read the sub-spectrum
     }
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
     if ((tid & 63) == 0) red[tid >> 6] = mx;
     __syncthreads();
     if (tid == 0) {
         for (int w = 1; w < NT / 64; ++w) mx = fmax(mx, red[w]);
-        partial[blockIdx.x] = sqrt(mx);
+        peak[line] = sqrt(mx);
     }
 }
 
@@ -76,7 +90,7 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict__ in, int64_t ld_in,
                                                         cpx<T>* __restrict__ out, int64_t ld_out, int N, int fsz,
                                                         int stride, int nf, int wpc, int nchunk, double det,
-                                                        double mit, const T* __restrict__ partial, int r,
+                                                        double mit, const T* __restrict__ peak,
                                                         uint8_t* __restrict__ flags, FftPlan pw,
                                                         const cpx<T>* __restrict__ tw,
                                                         const int32_t* __restrict__ perm) {
@@ -85,9 +99,7 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
     const int tid = threadIdx.x;
     const cpx<T>* row = in + (int64_t)line * ld_in;
     cpx<T>* orow = out + (int64_t)line * ld_out;
-    double mx = 0;
-    for (int q = 0; q < r; ++q) mx = fmax(mx, (double)partial[line * r + q]);
-    const bool flagged = !(mx < det);
+    const bool flagged = !((double)peak[line] < det);
     if (flags && chunk == 0 && tid == 0) flags[line] = flagged ? 1 : 0;
     const int s0 = chunk * wpc * stride;
     const int s1 = min(s0 + wpc * stride, N);
@@ -143,16 +155,26 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
     }
 }
 
+// LDS of the detection kernel for sub-length n1 = N / r.
+static size_t detect_lds_bytes(size_t es, int n1, int r) { return es * (size_t)(n1 + kTwLo + tw_hi_count(n1) + r); }
+
 template <typename T>
 static int run_eccm(hipStream_t s, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,
                     const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
-                    void* partial) {
+                    void* peak) {
     constexpr int NT = 512;
     const int L = d.n_lines, N = d.n_samples;
-    const size_t lds_det = sizeof(cpx<T>) * (size_t)(N / r);
-    hipLaunchKernelGGL((eccm_detect_kernel<T, NT>), dim3((unsigned)(L * r)), dim3(NT), lds_det, s,
-                       (const cpx<T>*)in, ld_in, N, r, fsub.plan, tw_of<T>(fN), tw_of<T>(fsub), fsub.perm,
-                       (T*)partial);
+    const size_t lds_det = detect_lds_bytes(sizeof(cpx<T>), N / r, r);
+#define S1G_DETECT(CLS)                                                                                        \
+    hipLaunchKernelGGL((eccm_detect_kernel<T, NT, kRadixClassMax[CLS]>), dim3((unsigned)L), dim3(NT), lds_det, s, \
+                       (const cpx<T>*)in, ld_in, N, r, fsub.plan, tw_of<T>(fN), tw_of<T>(fsub), (T*)peak)
+    switch (radix_class(fsub.plan)) {
+        case 0: S1G_DETECT(0); break;
+        case 1: S1G_DETECT(1); break;
+        case 2: S1G_DETECT(2); break;
+        default: S1G_DETECT(3); break;
+    }
+#undef S1G_DETECT
     const int nf = (int)floor(((double)N / (double)d.fft_size) * ((double)d.fft_size / (double)d.stride));
     const int wpc = 16;
     const int nchunk = (N + wpc * d.stride - 1) / (wpc * d.stride);
@@ -160,9 +182,22 @@ static int run_eccm(hipStream_t s, const s1g_eccm_desc& d, const DevFft& fN, con
     const size_t lds = (sizeof(cpx<T>) + 1) * (size_t)nw_max * d.fft_size;
     hipLaunchKernelGGL((eccm_notch_kernel<T, NT>), dim3((unsigned)(L * nchunk)), dim3(NT), lds, s,
                        (const cpx<T>*)in, ld_in, (cpx<T>*)out, ld_out, N, d.fft_size, d.stride, nf > 0 ? nf : 0, wpc,
-                       nchunk, d.detection_threshold, d.mitigation_threshold, (const T*)partial, r, flags, fw.plan,
+                       nchunk, d.detection_threshold, d.mitigation_threshold, (const T*)peak, flags, fw.plan,
                        tw_of<T>(fw), fw.perm);
     return (int)hipGetLastError();
+}
+
+int eccm_detect_split(int prec, int N, bool (*factor)(int, FftPlan&)) {
+    // the smallest r with N / r factoring over the pass radices and the
+    // workgroup's LDS within 52 KB (three workgroups per CU)
+    static const int cand[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40, 48, 64};
+    const size_t es = prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+    for (int k : cand) {
+        if (N % k) continue;
+        FftPlan p;
+        if (detect_lds_bytes(es, N / k, k) <= 52 * 1024 && factor(N / k, p)) return k;
+    }
+    return 0;
 }
 
 int launch_eccm(hipStream_t s, int prec, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,

@@ -180,16 +180,25 @@ int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, cons
 // ECCM.
 // Out of place (windows overlap, so workgroups would race in place).
 // fN: plan of length N (twiddles only), fsub: plan of N / r, fw: window plan;
-// partial: n_lines * r values of the detection maxima.
+// partial: n_lines values of the detection maxima.
 int launch_eccm(hipStream_t stream, int prec, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,
                 const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
                 void* partial);
+// Detection split r for lines of N samples (0: none), given the planner.
+int eccm_detect_split(int prec, int N, bool (*factor)(int, FftPlan&));
 
 // General signal processing (dsp.hip).
 int launch_copy_scale(hipStream_t stream, int prec, const void* src, int64_t lds, void* dst, int64_t ldd, int rows,
                       int n, double scale);
 int launch_transpose(hipStream_t stream, int prec, const void* src, int64_t lds, void* dst, int64_t ldd, int rows,
                      int cols);
+// Rows that fit one workgroup's LDS (at most kRowLdsBytes per row): in-place
+// DFT of `rows` rows of length f.plan.n, one workgroup per row, the result
+// multiplied by `scale`.  -1 when the plan or the length does not qualify.
+constexpr size_t kRowLdsBytes = 160 * 1024;
+bool fft_rows_lds_fits(int prec, int n);
+int launch_fft_rows_lds(hipStream_t stream, int prec, const DevFft& f, void* data, int64_t ld, int rows, int inverse,
+                        double scale);
 // Bluestein: filter spectrum for an n-point transform through M points
 // (scratch 3 M complex doubles), then the transform of `rows` rows of x in
 // place (w1..w3: rows * M complex each); `scale` multiplies the result.

@@ -27,7 +27,9 @@ def _rand(shape, seed):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-@pytest.mark.parametrize("n", [1, 7, 256, 1480, 1867, 14936, 30744])
+# 1..19608: whole rows in LDS (FP64 up to 10,240 points); 24000 and 30744
+# (and FP64 19608): global passes; 1867 and 14936: Bluestein.
+@pytest.mark.parametrize("n", [1, 7, 256, 1480, 1867, 4902, 14936, 19608, 24000, 30744])
 @pytest.mark.parametrize("prec", [F64, F32])
 def test_dft_rows(ctx, n, prec):
     x = _rand((3, n), n)
