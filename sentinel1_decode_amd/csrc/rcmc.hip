@@ -528,6 +528,13 @@ __device__ __forceinline__ void migration_terms(double u, double& idm1, double& 
 // (at most a few samples at Sentinel-1 geometries).
 constexpr int kLinSlack = 64;
 
+// LDS position of tile element a: one pad element per 16.  A thread's run
+// starts EPT = 8 elements after its neighbour's, a 64-byte lane stride that
+// would put the lanes of one access on a few of the 32 banks (the LDS was
+// stalled by bank conflicts for a third of the kernel); the pad rotates
+// each 128-byte row of the tile by one element.
+__device__ __forceinline__ int lin_pad(int a) { return a + (a >> 4); }
+
 // FP32 chain, linear column geometry (AzUniform::ok).  With v_rel and f_dc
 // the same in every column, D, 1/D - 1 and 1 - D depend on the Doppler row
 // only; R_j / dR, bt and t_corr are linear in j, so the migration is
@@ -551,7 +558,7 @@ __global__ __launch_bounds__(NT) void rcmc32_lin_kernel(RcmcBatch bt) {
     constexpr int PER = NT * EPT;
     constexpr int CAP = PER + 2 * kTaps + kLinSlack;
     constexpr int NW = 2 * kTaps + EPT;
-    __shared__ cpx<float> tile[CAP];
+    __shared__ cpx<float> tile[CAP + CAP / 16 + 1];  // indexed through lin_pad
     // the workgroup's instance (uniform: a scan of the kernel arguments)
     int kI = 0;
     while (kI + 1 < bt.n && (int)blockIdx.x >= bt.inst[kI + 1].wg0) ++kI;
@@ -609,8 +616,21 @@ __global__ __launch_bounds__(NT) void rcmc32_lin_kernel(RcmcBatch bt) {
         const int tlo = j0 + (int)floor(fmin(shA, shB)) - kTaps;
         const int thi = jl + (int)floor(fmax(shA, shB)) + kTaps;
         const bool use_tile = rcmc && thi - tlo + 1 <= CAP;
-        if (use_tile)
-            for (int k = tlo + tid; k <= thi; k += NT) tile[k - tlo] = (k >= 0 && k < N) ? row[k] : mk(0.0f, 0.0f);
+        if (use_tile) {
+            // every load of the tile issued before the first LDS store (raw
+            // buffer loads: offsets outside the row read zero, the padding)
+            constexpr int NLD = (CAP + NT - 1) / NT;
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(row, (uint32_t)N * (uint32_t)sizeof(cpx<float>));
+            cpx<float> v[NLD];
+#pragma unroll
+            for (int u = 0; u < NLD; ++u) {
+                const int k = tlo + tid + NT * u;
+                v[u] = buf_load<float>(rs, (k >= 0 && k <= thi) ? (uint32_t)k * (uint32_t)sizeof(cpx<float>) : kBufSkip);
+            }
+#pragma unroll
+            for (int u = 0; u < NLD; ++u)
+                if (tid + NT * u <= thi - tlo) tile[lin_pad(tid + NT * u)] = v[u];
+        }
         __syncthreads();
         if (js < N) {
             const double s0 = (g.rdr0 + js * g.drdr) * idm1;
@@ -626,10 +646,10 @@ __global__ __launch_bounds__(NT) void rcmc32_lin_kernel(RcmcBatch bt) {
                 sinc_weights((float)(s1 - f0), dw);
 #pragma unroll
                 for (int k = 0; k <= 2 * kTaps; ++k) dw[k] -= w0[k];
-                const cpx<float>* tp = tile + (js + (int)f0 - kTaps - tlo);
+                const int tb = js + (int)f0 - kTaps - tlo;
                 cpx<float> x[NW];
 #pragma unroll
-                for (int m = 0; m < NW; ++m) x[m] = tp[m];
+                for (int m = 0; m < NW; ++m) x[m] = tile[lin_pad(tb + m)];
 #pragma unroll
                 for (int e = 0; e < EPT; ++e) {
                     const float t = (float)e / (float)(EPT - 1);
@@ -664,12 +684,12 @@ __global__ __launch_bounds__(NT) void rcmc32_lin_kernel(RcmcBatch bt) {
         // store consecutive samples (whole lines, no partial-line writes)
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) tile[tid * EPT + e] = acc[e];
+        for (int e = 0; e < EPT; ++e) tile[lin_pad(tid * EPT + e)] = acc[e];
         __syncthreads();
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
             const int q = e * NT + tid;
-            if (j0 + q < N) orow[j0 + q] = tile[q];
+            if (j0 + q < N) orow[j0 + q] = tile[lin_pad(q)];
         }
     } else if (js < N) {
 #pragma unroll
