@@ -57,7 +57,24 @@ __global__ __launch_bounds__(NT, MINW) void fft_rows_lds_kernel(cpx<T>* __restri
     if (row >= rows) return;
     cpx<T>* x = data + (int64_t)row * ld;
     load_tw2(tw, n, lo, hi, tid, NT);
-    for (int q = tid; q < n; q += NT) buf[perm[q]] = x[q];
+    // loads in batches of kRowBatch per thread, all issued before the
+    // batch's LDS stores (one memory wait per batch, not per element)
+    constexpr int kRowBatch = 8;
+    for (int q0 = 0; q0 < n; q0 += NT * kRowBatch) {
+        cpx<T> v[kRowBatch];
+        int pos[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            const int q = q0 + tid + NT * u;
+            if (q < n) {
+                v[u] = x[q];
+                pos[u] = perm[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u)
+            if (q0 + tid + NT * u < n) buf[pos[u]] = v[u];
+    }
     __syncthreads();
     fft_dit_lds_tw2<T, INV, MAXR>(buf, plan, lo, hi, tid, NT);
     for (int q = tid; q < n; q += NT) x[q] = scl(buf[q], scale);

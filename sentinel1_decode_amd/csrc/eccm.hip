@@ -45,16 +45,37 @@ __global__ __launch_bounds__(NT) void eccm_detect_kernel(const cpx<T>* __restric
     __syncthreads();
     T mx = 0;
     for (int q = 0; q < r; ++q) {
-        for (int n = tid; n < n1; n += NT) {
-            cpx<T> y = row[n];
+        // four elements per thread at a time, their loads issued together
+        for (int n0 = 0; n0 < n1; n0 += 4 * NT) {
+            cpx<T> y[4], t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int n = n0 + tid + NT * u;
+                y[u] = mk((T)0, (T)0);
+                t[u] = mk((T)1, (T)0);
+                if (n < n1) {
+                    y[u] = row[n];
+                    if (q) t[u] = twN[q * n];
+                }
+            }
             int k = 0;  // (q m) mod r
             for (int m = 1; m < r; ++m) {
                 k += q;
                 if (k >= r) k -= r;
-                y = y + cmul(row[n + m * n1], wr[k]);
+                cpx<T> x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int n = n0 + tid + NT * u;
+                    x[u] = n < n1 ? row[n + m * n1] : mk((T)0, (T)0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) y[u] = y[u] + cmul(x[u], wr[k]);
             }
-            if (q) y = cmul(y, twN[q * n]);
-            buf[n] = y;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int n = n0 + tid + NT * u;
+                if (n < n1) buf[n] = q ? cmul(y[u], t[u]) : y[u];
+            }
         }
         __syncthreads();
         fft_dif_lds_tw2<T, false, MAXR>(buf, sub, lo, hi, tid, NT);
@@ -113,14 +134,28 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
     const int nw = w1 - w0;
     cpx<T>* buf = (cpx<T>*)smem_raw;                       // nw * fsz
     uint8_t* keep = (uint8_t*)(buf + (size_t)nw * fsz);    // nw * fsz
-    for (int idx = tid; idx < nw * fsz; idx += NT) {
-        const int w = idx / fsz, j = idx - w * fsz;
-        const int s = (w0 + w) * stride;
-        int e = s + fsz;
-        if (e > N) e = N - 1;
-        cpx<T> v = mk((T)0, (T)0);
-        if (s + j < e) v = row[s + j];
-        buf[w * fsz + perm[j]] = v;
+    // the window samples, loaded four per thread at a time before their LDS
+    // stores (one memory wait per group)
+    for (int i0 = 0; i0 < nw * fsz; i0 += 4 * NT) {
+        cpx<T> v[4];
+        int pos[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = i0 + tid + NT * u;
+            const int w = idx / fsz, j = idx - w * fsz;
+            const int s = (w0 + w) * stride;
+            int e = s + fsz;
+            if (e > N) e = N - 1;
+            v[u] = mk((T)0, (T)0);
+            pos[u] = -1;
+            if (idx < nw * fsz) {
+                pos[u] = w * fsz + perm[j];
+                if (s + j < e) v[u] = row[s + j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (pos[u] >= 0) buf[pos[u]] = v[u];
     }
     __syncthreads();
     fft_dit_lds<T, false, true>(buf, pw, nw, fsz, 1, tw, tid, NT);

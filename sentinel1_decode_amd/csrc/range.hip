@@ -189,11 +189,17 @@ __global__ __launch_bounds__(NT, (sizeof(T) == 4 ? 2 : 1)) void range_os3_kernel
             a[n0] = (src >= 0 && src < N) ? row[src] : mk((T)0, (T)0);
         }
     }
-    for (int i = tid; i < 128; i += NT) tlo[i] = tw[i];
-    for (int i = tid; i < NHI; i += NT) thi[i] = tw[128 * i];
-    for (int i = tid; i < R1 * S1; i += NT) {
-        const int k1 = i / S1, r = i - k1 * S1;
-        t1[i] = tw[((r * k1) % S0) * R0];
+    // twiddle tables: every load issued (beside the data loads) before the
+    // first LDS store, so the workgroup waits for memory once
+    {
+        static_assert(128 <= NT && NHI <= NT && R1 * S1 <= NT, "one table entry per thread");
+        const cpx<T> vlo = tid < 128 ? tw[tid] : mk((T)0, (T)0);
+        const cpx<T> vhi = tid < NHI ? tw[128 * tid] : mk((T)0, (T)0);
+        const int k1 = tid / S1, r = tid - k1 * S1;
+        const cpx<T> v1 = tid < R1 * S1 ? tw[((r * k1) % S0) * R0] : mk((T)0, (T)0);
+        if (tid < 128) tlo[tid] = vlo;
+        if (tid < NHI) thi[tid] = vhi;
+        if (tid < R1 * S1) t1[tid] = v1;
     }
     __syncthreads();
 
