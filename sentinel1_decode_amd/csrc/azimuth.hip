@@ -241,6 +241,12 @@ __global__ __launch_bounds__(NT) void az_fft_reg_kernel(AzBatch bt, const cpx<T>
     constexpr int QF = (LZ * W + NT - 1) / NT;       // first-pass butterflies per thread
     constexpr int LA = L / RA;                       // last-pass butterflies per column
     constexpr int QL = (LA * W + NT - 1) / NT;
+    // Long odd last-pass butterflies are split over two lanes when whole
+    // waves can take one half each (SPLIT): the pass's time is that of the
+    // SIMD with the most butterfly waves, and LA W = 320 butterflies (five
+    // waves) on eight waves would leave two full butterflies on one SIMD.
+    constexpr bool SPLIT = RA >= 31 && RA % 2 == 1 && (LA * W) % 64 == 0;
+    constexpr int QS = (2 * LA * W + NT - 1) / NT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;        // L * W
     cpx<T>* tw = buf + (size_t)L * W;       // L
@@ -301,6 +307,37 @@ __global__ __launch_bounds__(NT) void az_fft_reg_kernel(AzBatch bt, const cpx<T>
         const int j0 = P.j0, w = P.w;
         const int64_t ld = P.ld;
         const auto rsrc = P.rsrc;
+        if constexpr (SPLIT) {
+            // item i < LA W: the DC term and pairs 1 .. HS of butterfly i;
+            // item LA W + i: pairs HS + 1 .. (RA - 1) / 2 of butterfly i.
+            // LA W is a multiple of 64, so each wave takes one kind, and
+            // waves past the last item skip the pass.
+            constexpr int HS = (RA - 1) / 4;
+#pragma unroll
+            for (int q = 0; q < QS; ++q) {
+                const int item = tid + NT * q;
+                if (item < 2 * LA * W) {
+                    const bool second = item >= LA * W;
+                    const int idx = second ? item - LA * W : item;
+                    const int v = idx % W;
+                    const bool ok = v < w;
+                    const int g = idx / W;
+                    cpx<T> a[RA];
+#pragma unroll
+                    for (int rr = 0; rr < RA; ++rr) a[rr] = buf[(g * RA + rr) * W + v];
+                    auto sink = [&](int k, cpx<T> y) {
+                        int row = ipos[g * RA + k] - sh;
+                        if (row < 0) row += L;
+                        const uint32_t off =
+                            ok ? (uint32_t)(((int64_t)row * ld + j0 + v) * (int64_t)sizeof(cpx<T>)) : kBufSkip;
+                        buf_store(INV ? scl(y, scale) : y, rsrc, off);
+                    };
+                    if (second) Dft<T, RA, INV>::template run_sink_part<HS + 1, (RA - 1) / 2 + 1, false>(a, sink);
+                    else Dft<T, RA, INV>::template run_sink_part<1, HS + 1, true>(a, sink);
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < QL; ++q) {
             const int idx = tid + NT * q;

@@ -228,10 +228,43 @@ __device__ __forceinline__ void odd_out_sink(const cpx<T>* s, const cpx<T>* d, c
     }
 }
 
+// Pairs K0 .. K1 - 1 of odd_out_sink only.
+template <typename T, int R, bool INV, int K, int K1, typename Sink>
+__device__ __forceinline__ void odd_out_sink_range(const cpx<T>* s, const cpx<T>* d, cpx<T> x0, Sink& sink) {
+    if constexpr (K < K1) {
+        T ar = x0.x, ai = x0.y, br = 0, bi = 0;
+        odd_acc<T, R, K, 1>(s, d, ar, ai, br, bi);
+        if (INV) {
+            sink(K, mk(ar - bi, ai + br));
+            sink(R - K, mk(ar + bi, ai - br));
+        } else {
+            sink(K, mk(ar + bi, ai - br));
+            sink(R - K, mk(ar - bi, ai + br));
+        }
+        odd_out_sink_range<T, R, INV, K + 1, K1>(s, d, x0, sink);
+    }
+}
+
 template <typename T, int R, bool INV>
 struct Dft {
     static_assert(R % 2 == 1, "generic butterfly handles odd lengths");
     static constexpr int H = (R - 1) / 2;
+    // Part of the outputs: X_0 when DC, and the pairs (k, R - k) for
+    // K0 <= k < K1 (two lanes can share one butterfly this way).
+    template <int K0, int K1, bool DC, typename Sink>
+    static __device__ __forceinline__ void run_sink_part(const cpx<T>* a, Sink& sink) {
+        cpx<T> s[H], d[H];
+        const cpx<T> x0 = a[0];
+        cpx<T> sum = a[0];
+#pragma unroll
+        for (int m = 1; m <= H; ++m) {
+            s[m - 1] = a[m] + a[R - m];
+            d[m - 1] = a[m] - a[R - m];
+            if constexpr (DC) sum = sum + s[m - 1];
+        }
+        if constexpr (DC) sink(0, sum);
+        odd_out_sink_range<T, R, INV, K0, K1>(s, d, x0, sink);
+    }
     template <typename Sink>
     static __device__ __forceinline__ void run_sink(const cpx<T>* a, Sink& sink) {
         cpx<T> s[H], d[H];
