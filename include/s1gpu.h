@@ -573,7 +573,8 @@ int s1g_af_adjoint(s1g_ctx* ctx, int prec, int n_lines, int n_samples, const voi
 #define S1G_STAGE_AZ_IFFT 5
 #define S1G_STAGE_FILTER 6
 #define S1G_STAGE_AUTOFOCUS 7
-#define S1G_NSTAGES 8
+#define S1G_STAGE_DFT 8
+#define S1G_NSTAGES 9
 int s1g_profile_enable(s1g_ctx* ctx, int on);
 int s1g_profile_read(s1g_ctx* ctx, double* ms, int64_t* launches, int reset);
 
@@ -581,10 +582,11 @@ int s1g_profile_read(s1g_ctx* ctx, double* ms, int64_t* launches, int reset);
 /* (No reference counterpart.)  Stages with several launch variants --
  * S1G_STAGE_RANGE (overlap-save block length), S1G_STAGE_AZ_FFT and
  * S1G_STAGE_AZ_IFFT (panel width, compiled radix sequence or generic
- * kernel), S1G_STAGE_RCMC_MF (workgroup tile) -- time every candidate on
- * the first eager call for a shape (the in-place azimuth FFT on a scratch
- * array), keep the fastest in the context, and reuse it; all variants give
- * the same result to rounding.  Launches recorded into a graph use the
+ * kernel), S1G_STAGE_RCMC_MF (workgroup tile), S1G_STAGE_DFT (the radix
+ * order of s1g_dft's rows that fit one workgroup's LDS; its timer slot stays
+ * empty) -- time every candidate on the first eager call for a shape (the
+ * in-place transforms on scratch arrays), keep the fastest in the context,
+ * and reuse it; all variants give the same result to rounding.  Launches recorded into a graph use the
  * cached choice; s1g_focus_batch runs the chain once eagerly before it
  * captures.  Autotuning is on by default. */
 int s1g_set_autotune(s1g_ctx* ctx, int on);

@@ -159,7 +159,7 @@ _SIGS = [
     ("s1g_last_variant", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
-STAGES = ["decode", "eccm", "range", "az_fft", "rcmc_mf", "az_ifft", "filter", "autofocus"]
+STAGES = ["decode", "eccm", "range", "az_fft", "rcmc_mf", "az_ifft", "filter", "autofocus", "dft"]
 
 _LIB = None
 

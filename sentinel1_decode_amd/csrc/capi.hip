@@ -43,6 +43,7 @@ struct s1g_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::map<int, DevFft> ffts;
+    std::map<std::pair<int, int>, DevFft> fft_orders;  // (n, order) -> plan with reordered radices
     std::map<FilterKey, void*> filters;
     void* filter_scratch = nullptr;
     size_t filter_scratch_bytes = 0;
@@ -94,7 +95,7 @@ struct s1g_ctx {
     size_t aux_bytes[16] = {};
     // autotuner
     bool autotune = true;
-    int force[S1G_NSTAGES] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    int force[S1G_NSTAGES] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
     int last_var[S1G_NSTAGES] = {0};
     int last_nvar[S1G_NSTAGES] = {0};
     struct Tuned {
@@ -186,6 +187,24 @@ bool factorise(int n, FftPlan& p) {
     return true;
 }
 
+// Digit-reversed position of input i for DIT passes radix[0..S-1].
+std::vector<int32_t> digit_reversal(const FftPlan& p) {
+    const int n = p.n;
+    std::vector<int32_t> perm(n);
+    for (int i = 0; i < n; ++i) {
+        int64_t pos = i, rem = i, m = n;
+        if (p.npass > 0) pos = 0;
+        for (int s = p.npass - 1; s >= 0; --s) {
+            const int R = p.radix[s];
+            m /= R;
+            pos += (rem % R) * m;
+            rem /= R;
+        }
+        perm[i] = (int32_t)pos;
+    }
+    return perm;
+}
+
 int get_fft(s1g_ctx* c, int n, const DevFft** out) {
     auto it = c->ffts.find(n);
     if (it != c->ffts.end()) {
@@ -210,19 +229,7 @@ int get_fft(s1g_ctx* c, int n, const DevFft** out) {
         t32[k].x = (float)cr;
         t32[k].y = (float)ci;
     }
-    // digit-reversed position of input i for DIT passes radix[0..S-1]
-    std::vector<int32_t> perm(n);
-    for (int i = 0; i < n; ++i) {
-        int64_t pos = i, rem = i, m = n;
-        if (f.plan.npass > 0) pos = 0;
-        for (int s = f.plan.npass - 1; s >= 0; --s) {
-            const int R = f.plan.radix[s];
-            m /= R;
-            pos += (rem % R) * m;
-            rem /= R;
-        }
-        perm[i] = (int32_t)pos;
-    }
+    const std::vector<int32_t> perm = digit_reversal(f.plan);
     HIPCHK(c, hipMalloc(&f.tw64, sizeof(cpx<double>) * n));
     HIPCHK(c, hipMalloc(&f.tw32, sizeof(cpx<float>) * n));
     HIPCHK(c, hipMalloc(&f.perm, sizeof(int32_t) * n));
@@ -230,6 +237,68 @@ int get_fft(s1g_ctx* c, int n, const DevFft** out) {
     HIPCHK(c, hipMemcpy(f.tw32, t32.data(), sizeof(cpx<float>) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(f.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     auto ins = c->ffts.emplace(n, f);
+    *out = &ins.first->second;
+    return 0;
+}
+
+// Radix orders the DFT tuner tries for a plan (the planner's own order
+// first): the same radices reversed, ascending, and with the powers of two
+// first; orders equal to an earlier one are dropped.  Any order is a valid
+// plan; it changes the passes' strides and twiddle patterns.
+int radix_orders(const FftPlan& p, FftPlan* out, int max_orders) {
+    std::vector<std::vector<int>> seen;
+    auto add = [&](std::vector<int> r) {
+        for (const auto& q : seen)
+            if (q == r) return;
+        seen.push_back(r);
+    };
+    std::vector<int> base(p.radix, p.radix + p.npass);
+    add(base);
+    std::vector<int> rev(base.rbegin(), base.rend());
+    add(rev);
+    std::vector<int> asc = base;
+    std::sort(asc.begin(), asc.end());
+    add(asc);
+    std::vector<int> pow2, odd;
+    for (int r : base) ((r & (r - 1)) == 0 ? pow2 : odd).push_back(r);
+    pow2.insert(pow2.end(), odd.begin(), odd.end());
+    add(pow2);
+    int k = 0;
+    for (const auto& r : seen) {
+        if (k == max_orders) break;
+        out[k].n = p.n;
+        out[k].npass = (int)r.size();
+        for (size_t i = 0; i < r.size(); ++i) out[k].radix[i] = r[i];
+        ++k;
+    }
+    return k;
+}
+
+// The plan of length n with its radices in order `order` of radix_orders
+// (order 0: the default plan), sharing the default plan's twiddle tables.
+int get_fft_order(s1g_ctx* c, int n, int order, const DevFft** out) {
+    const DevFft* f0;
+    int rc = get_fft(c, n, &f0);
+    if (rc || order == 0 || f0->plan.npass < 0) {
+        *out = f0;
+        return rc;
+    }
+    auto it = c->fft_orders.find({n, order});
+    if (it != c->fft_orders.end()) {
+        *out = &it->second;
+        return 0;
+    }
+    FftPlan ord[4];
+    const int no = radix_orders(f0->plan, ord, 4);
+    if (order >= no) return fail(c, S1G_EARG, "no such radix order%s");
+    DevFft f;
+    f.plan = ord[order];
+    f.tw64 = f0->tw64;
+    f.tw32 = f0->tw32;
+    const std::vector<int32_t> perm = digit_reversal(f.plan);
+    HIPCHK(c, hipMalloc(&f.perm, sizeof(int32_t) * n));
+    HIPCHK(c, hipMemcpy(f.perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    auto ins = c->fft_orders.emplace(std::make_pair(n, order), f);
     *out = &ins.first->second;
     return 0;
 }
@@ -576,6 +645,7 @@ int s1g_destroy(s1g_ctx* c) {
         hipFree(kv.second.tw32);
         hipFree(kv.second.perm);
     }
+    for (auto& kv : c->fft_orders) hipFree(kv.second.perm);  // twiddles are the default plan's
     for (auto& kv : c->filters) hipFree(kv.second);
     hipFree(c->filter_scratch);
     hipFree(c->ws);
@@ -933,8 +1003,26 @@ static int dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t l
     const size_t es = esize(prec);
     const double scale = inverse && normalise ? 1.0 / (double)n : 1.0;
     if (f->plan.npass >= 0 && fft_rows_lds_fits(prec, n)) {
-        // whole rows in LDS: one read and one write per row
-        rc = launch_fft_rows_lds(c->stream, prec, *f, data, ld, rows, inverse, scale);
+        // whole rows in LDS: one read and one write per row; the radix order
+        // is the tuner's (its trials transform up to 2,048 rows of the
+        // scratch buffer, whatever it holds, so the caller's data is
+        // transformed once)
+        FftPlan ord[4];
+        const int nvar = radix_orders(f->plan, ord, 4);
+        const int trial_rows = std::min(rows, 2048);
+        auto run = [&](int v) {
+            const DevFft* fv;
+            int r = get_fft_order(c, n, v, &fv);
+            if (!r) r = ensure(c, &c->dft_ws, &c->dft_ws_bytes, es * (size_t)trial_rows * n);
+            if (r) return r;
+            return check_launch(c, launch_fft_rows_lds(c->stream, prec, *fv, c->dft_ws, n, trial_rows, inverse, scale),
+                                "dft");
+        };
+        const int v = choose_variant(c, TuneKey{S1G_STAGE_DFT, prec, n, inverse, 0}, nvar, run);
+        const DevFft* fv;
+        rc = get_fft_order(c, n, v, &fv);
+        if (rc) return rc;
+        rc = launch_fft_rows_lds(c->stream, prec, *fv, data, ld, rows, inverse, scale);
         return check_launch(c, rc, "dft");
     }
     if (f->plan.npass >= 0) {

@@ -235,3 +235,27 @@ def test_tuning_cache_file(tmp_path):
     with pytest.raises(capi.S1GError):
         b.tune_load(str(bad))
     b.close()
+
+
+@pytest.mark.parametrize("n", [1480, 4902])
+@pytest.mark.parametrize("prec,tol", [(F64, 1e-12), (F32, 3e-6)])
+def test_dft_radix_orders(vctx, n, prec, tol):
+    """Every radix order the DFT tuner can pick for rows that fit LDS gives
+    numpy's transform (1480 = 37 * 5 * 8: four orders; 4902 = 43 * 19 * 3 * 2:
+    three distinct ones)."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((5, n)) + 1j * rng.standard_normal((5, n))
+    dt = torch.complex128 if prec == F64 else torch.complex64
+    x0 = torch.from_numpy(x).to(dt).cuda()
+    nv = _variant_count(vctx, "dft", lambda: vctx.dft(x0.clone(), axis=1))
+    assert nv == (4 if n == 1480 else 3)
+    for inverse in (False, True):
+        ref = np.fft.ifft(x, axis=1) if inverse else np.fft.fft(x, axis=1)
+        for v in range(nv):
+            vctx.force_variant("dft", v)
+            d = x0.clone()
+            vctx.dft(d, axis=1, inverse=inverse)
+            torch.cuda.synchronize()
+            assert vctx.last_variant("dft") == (v, nv)
+            assert rel_l2(d.cpu().numpy(), ref) < tol, (v, inverse)
+    vctx.force_variant("dft", -1)
