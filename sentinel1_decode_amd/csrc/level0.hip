@@ -210,7 +210,7 @@ __global__ void src_kernel(cpx<T>* __restrict__ data, int64_t ld, int n_rows, in
     k /= (kC * R[j] * (f * f));
     const double ph = -1.0 * kPI * (fr[j] * fr[j]) / k;
     double s, c;
-    sincos(ph, &s, &c);
+    phase_sincos<T>(ph, &s, &c);
     cpx<T>& x = data[(int64_t)i * ld + j];
     const double xr = (double)x.x, xi = (double)x.y;
     x.x = (T)(xr * c - xi * s);
@@ -228,7 +228,7 @@ __global__ void rcmc_phase_kernel(cpx<T>* __restrict__ data, int64_t ld, int n_r
     const double shift = (R[j] / factors[(int64_t)i * n + j]) - R[j];
     const double ph = -4.0 * kPI * fr[j] * shift / kC;
     double s, c;
-    sincos(ph, &s, &c);
+    phase_sincos<T>(ph, &s, &c);
     cpx<T>& x = data[(int64_t)i * ld + j];
     const double xr = (double)x.x, xi = (double)x.y;
     x.x = (T)(xr * c - xi * s);

@@ -522,6 +522,29 @@ __device__ __forceinline__ void fft_dif_lds(cpx<T>* buf, const FftPlan& p, int n
 }
 
 // ------------------------------------------------------- FP64 phase helpers
+// e^{i ph} for a phase formed in FP64.  The FP64 chain takes FP64 sincos
+// (the reference's arithmetic); the FP32 chain reduces the phase to a
+// fraction of a turn in FP64 and takes the FP32 hardware sine and cosine of
+// it, as its RCMC does, so phases of 1e8 rad lose nothing before the FP32
+// trig.
+template <typename T>
+__device__ __forceinline__ void phase_sincos(double ph, double* s, double* c) {
+    if constexpr (sizeof(T) == 8) {
+        sincos(ph, s, c);
+    } else {
+        const double t = ph * 0.15915494309189535;  // turns
+        const float f = (float)(t - rint(t));
+        *s = (double)__builtin_amdgcn_sinf(f);
+        *c = (double)__builtin_amdgcn_cosf(f);
+    }
+}
+
+// x^3 as the reference writes it (pow) in FP64, a product in the FP32 chain.
+template <typename T>
+__device__ __forceinline__ double cube(double x) {
+    if constexpr (sizeof(T) == 8) return pow(x, 3.0);
+    else return x * x * x;
+}
 // Reduction of a large FP64 phase to [-pi, pi] (Cody & Waite 1980 additive
 // reduction): 2 pi = C1 + C2 + C3 with C1, C2 holding 27 significant bits, so
 // k*C1 and k*C2 are exact for |k| < 2^26 (|phase| < 4e8 rad).

@@ -334,23 +334,24 @@ __global__ void tops_filter_kernel(cpx<T>* __restrict__ data, int64_t ld, int ro
     const int64_t k = (int64_t)i * n + j;
     const double r = R[j], d = D[k], v = vrel[k];
     cpx<double> x = ld64(data[(int64_t)i * ld + j]);
-    const double amp = sqrt(pow(ref_range / r, 3.0));
+    const double amp = sqrt(cube<T>(ref_range / r));
     x.x /= amp;
     x.y /= amp;
     // apply_range_cell_migration_correction (image_formation.cpp:458-476)
     const double shift = (r / d) - r;
     double s, c;
-    sincos((((-4.0 * kPi) * fr[j]) * shift) / kC, &s, &c);
+    phase_sincos<T>((((-4.0 * kPi) * fr[j]) * shift) / kC, &s, &c);
     cpx<double> y;
     y.x = x.x * c - x.y * s;
     y.y = x.x * s + x.y * c;
     // exp(4 i pi R D / lambda) exp(2 i pi (f_a + f_dc) t_corr)
     double s1, c1, s2, c2;
-    sincos((((4.0 * kPi) * r) * d) / kWavelength, &s1, &c1);
-    sincos(((2.0 * kPi) * (faz[i] + fdc[j])) * tc[j], &s2, &c2);
+    phase_sincos<T>((((4.0 * kPi) * r) * d) / kWavelength, &s1, &c1);
+    phase_sincos<T>(((2.0 * kPi) * (faz[i] + fdc[j])) * tc[j], &s2, &c2);
     const double hr = c1 * c2 - s1 * s2, hi = c1 * s2 + s1 * c2;
     data[(int64_t)i * ld + j] = st<T>(y.x * hr - y.y * hi, y.x * hi + y.y * hr);
-    fm[k] = -2 * pow(v, 2.0) * pow(d, 3.0) / (kWavelength * r);
+    if constexpr (sizeof(T) == 8) fm[k] = -2 * pow(v, 2.0) * pow(d, 3.0) / (kWavelength * r);
+    else fm[k] = -2 * (v * v) * (d * d * d) / (kWavelength * r);
 }
 
 // ---------------------------------------------------- azimuth time UFR
@@ -374,7 +375,7 @@ __global__ void time_deramp_kernel(const cpx<T>* __restrict__ in, const double* 
     const double p1 = ((-1.0 * kPi) * kt) * (t * t);
     const double p2 = (((2.0 * kPi) * (kt + ka)) * dct) * t;
     double s, c;
-    sincos(p1 + p2, &s, &c);
+    phase_sincos<T>(p1 + p2, &s, &c);
     const cpx<double> x = ld64(in[(int64_t)r * n_az + (a % n_az)]);
     out[(int64_t)r * shape + a] = st<T>(c * x.x - s * x.y, c * x.y + s * x.x);
 }
@@ -405,7 +406,7 @@ __global__ void time_resample_kernel(const cpx<T>* __restrict__ in, int shape, c
     const double p1 = ((1.0 * kPi) * kt) * (t * t);
     const double p2 = (((2.0 * kPi) * (kt + ka)) * dct) * t;
     double s, c;
-    sincos(p1 - p2, &s, &c);
+    phase_sincos<T>(p1 - p2, &s, &c);
     out[(int64_t)r * n_out + k] = st<T>(c * vr - s * vi, c * vi + s * vr);
 }
 
