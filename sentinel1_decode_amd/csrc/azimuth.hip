@@ -4,6 +4,7 @@
 // s1_decoder.cpp:499-505) or the 1/L scale (inverse, s1_decoder.cpp:682,
 // signal_processing.cpp:581-605) folded in.
 #include "s1g_internal.h"
+#include "s1g_lds_fft.h"
 
 #include <algorithm>
 #include <initializer_list>
@@ -493,27 +494,44 @@ static bool az_fixed(hipStream_t s, const DevFft& fl, cpx<T>* data, int64_t ld, 
     }
 }
 
-template <typename T, bool INV, int NT>
+template <typename T, bool INV, int NT, int MAXR>
 __global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__ data, int64_t ld, int L, int N, int W,
                                                     int npanels, FftPlan plan, const cpx<T>* __restrict__ tw,
                                                     const int32_t* __restrict__ perm, T scale, int sh) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* buf = (cpx<T>*)smem_raw;
+    cpx<T>* lo = buf + (size_t)L * W;  // two-level twiddle tables
+    cpx<T>* hi = lo + kTwLo;
     const int panel = xcd_panel(blockIdx.x, npanels);
     if (panel >= npanels) return;
     const int j0 = panel * W;
     const int w = min(W, N - j0);
     const int tid = threadIdx.x;
     const int total = L * W;
-    if (!INV) {
-        for (int idx = tid; idx < total; idx += NT) {
+    load_tw2(tw, L, lo, hi, tid, NT);
+    // loads in batches of four per thread before their LDS stores
+    constexpr int B = 4;
+    for (int i0 = 0; i0 < total; i0 += NT * B) {
+        cpx<T> x[B];
+        int pos[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int idx = i0 + tid + NT * u;
             const int e = idx / W, v = idx - e * W;
-            cpx<T> x = mk((T)0, (T)0);
-            if (v < w) x = data[(int64_t)e * ld + j0 + v];
-            buf[perm[e] * W + v] = x;
+            x[u] = mk((T)0, (T)0);
+            pos[u] = -1;
+            if (idx < total) {
+                if (v < w) x[u] = data[(int64_t)e * ld + j0 + v];
+                pos[u] = (INV ? e : perm[e]) * W + v;
+            }
         }
-        __syncthreads();
-        fft_dit_lds<T, false>(buf, plan, W, 1, W, tw, tid, NT);
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (pos[u] >= 0) buf[pos[u]] = x[u];
+    }
+    __syncthreads();
+    if (!INV) {
+        fft_dit_lds_tw2v<T, false, MAXR>(buf, plan, W, 1, W, lo, hi, tid, NT);
         // sh = L - L/2 for the fftshift: out[i] = X[(i + L - L/2) mod L]
         for (int idx = tid; idx < total; idx += NT) {
             const int i = idx / W, v = idx - i * W;
@@ -522,14 +540,7 @@ __global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__
             if (v < w) data[(int64_t)i * ld + j0 + v] = buf[src * W + v];
         }
     } else {
-        for (int idx = tid; idx < total; idx += NT) {
-            const int e = idx / W, v = idx - e * W;
-            cpx<T> x = mk((T)0, (T)0);
-            if (v < w) x = data[(int64_t)e * ld + j0 + v];
-            buf[idx] = x;
-        }
-        __syncthreads();
-        fft_dif_lds<T, true>(buf, plan, W, 1, W, tw, tid, NT);
+        fft_dif_lds_tw2v<T, true, MAXR>(buf, plan, W, 1, W, lo, hi, tid, NT);
         for (int idx = tid; idx < total; idx += NT) {
             const int k = idx / W, v = idx - k * W;
             if (v < w) data[(int64_t)k * ld + j0 + v] = scl(buf[perm[k] * W + v], scale);
@@ -537,34 +548,42 @@ __global__ __launch_bounds__(NT) void az_fft_generic_kernel(cpx<T>* __restrict__
     }
 }
 
+template <typename T, bool INV>
+static void launch_az_generic_t(hipStream_t s, const DevFft& fl, T* data, int64_t ld, int L, int N, int W,
+                                int npanels, dim3 grid, size_t lds, T scale, int sh) {
+    constexpr int NT = 512;
+#define S1G_GEN(CLS)                                                                                            \
+    hipLaunchKernelGGL((az_fft_generic_kernel<T, INV, NT, kRadixClassMax[CLS]>), grid, dim3(NT), lds, s,          \
+                       (cpx<T>*)data, ld, L, N, W, npanels, fl.plan, tw_of<T>(fl), fl.perm, scale, sh)
+    switch (radix_class(fl.plan)) {
+        case 0: S1G_GEN(0); break;
+        case 1: S1G_GEN(1); break;
+        case 2: S1G_GEN(2); break;
+        default: S1G_GEN(3); break;
+    }
+#undef S1G_GEN
+}
+
 static int launch_az_generic(hipStream_t s, int prec, const DevFft& fl, void* data, int64_t ld, int L, int N,
                              int inverse, int sh) {
-    constexpr int NT = 512;
     // panel width: 64-byte row segments (8 complex64 / 4 complex128) within the LDS budget
     int W = prec == S1G_F32 ? 8 : 4;
     const size_t esz = prec == S1G_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
-    while (W > 1 && (size_t)L * W * esz > 150 * 1024) W /= 2;
-    if ((size_t)L * W * esz > 150 * 1024) return -3;
+    const size_t tabs = esz * (size_t)(kTwLo + tw_hi_count(L));
+    while (W > 1 && (size_t)L * W * esz + tabs > 150 * 1024) W /= 2;
+    if ((size_t)L * W * esz + tabs > 150 * 1024) return -3;
     const int npanels = (N + W - 1) / W;
     const int per = (npanels + 7) / 8;
     dim3 grid((unsigned)(8 * per));
-    const size_t lds = (size_t)L * W * esz;
+    const size_t lds = (size_t)L * W * esz + tabs;
     if (prec == S1G_F32) {
         const float sc = (float)(1.0 / (double)L);
-        if (inverse)
-            hipLaunchKernelGGL((az_fft_generic_kernel<float, true, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L, N,
-                               W, npanels, fl.plan, fl.tw32, fl.perm, sc, sh);
-        else
-            hipLaunchKernelGGL((az_fft_generic_kernel<float, false, NT>), grid, dim3(NT), lds, s, (cpx<float>*)data, ld, L,
-                               N, W, npanels, fl.plan, fl.tw32, fl.perm, 1.0f, sh);
+        if (inverse) launch_az_generic_t<float, true>(s, fl, (float*)data, ld, L, N, W, npanels, grid, lds, sc, sh);
+        else launch_az_generic_t<float, false>(s, fl, (float*)data, ld, L, N, W, npanels, grid, lds, 1.0f, sh);
     } else {
         const double sc = 1.0 / (L);
-        if (inverse)
-            hipLaunchKernelGGL((az_fft_generic_kernel<double, true, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
-                               N, W, npanels, fl.plan, fl.tw64, fl.perm, sc, sh);
-        else
-            hipLaunchKernelGGL((az_fft_generic_kernel<double, false, NT>), grid, dim3(NT), lds, s, (cpx<double>*)data, ld, L,
-                               N, W, npanels, fl.plan, fl.tw64, fl.perm, 1.0, sh);
+        if (inverse) launch_az_generic_t<double, true>(s, fl, (double*)data, ld, L, N, W, npanels, grid, lds, sc, sh);
+        else launch_az_generic_t<double, false>(s, fl, (double*)data, ld, L, N, W, npanels, grid, lds, 1.0, sh);
     }
     return (int)hipGetLastError();
 }

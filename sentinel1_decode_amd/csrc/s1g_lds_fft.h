@@ -158,4 +158,119 @@ __device__ __forceinline__ void fft_dif_lds_tw2(cpx<T>* buf, const FftPlan& p, c
     }
 }
 
+// Several vectors at once (element e of vector v at buf[v * vstride +
+// e * estride], as dit_pass / dif_pass in s1g_device.h) with the two-level
+// LDS twiddles.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dit_pass_tw2v(cpx<T>* buf, int n, int Ns, int nvec, int vstride, int estride,
+                                              const cpx<T>* lo, const cpx<T>* hi, int tid, int nthr) {
+    const int total = (n / R) * nvec;
+    const int tstep = n / (Ns * R);
+    const float inv_nvec = 1.0f / (float)nvec, inv_ns = 1.0f / (float)Ns;
+    for (int idx = tid; idx < total; idx += nthr) {
+        int v, j;
+        const int bf = fast_div(idx, nvec, inv_nvec, v);
+        const int g = fast_div(bf, Ns, inv_ns, j);
+        const int base = (g * Ns * R + j) * estride + v * vstride;
+        cpx<T> a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = buf[base + r * Ns * estride];
+        if (j) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const cpx<T> w = tw2(lo, hi, j * r * tstep);
+                a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+            }
+        }
+        if constexpr (R % 2 == 1 && R >= 11) {
+            auto sink = [&](int k, cpx<T> x) { buf[base + k * Ns * estride] = x; };
+            Dft<T, R, INV>::run_sink(a, sink);
+        } else {
+            Dft<T, R, INV>::run(a);
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+        }
+    }
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dif_pass_tw2v(cpx<T>* buf, int n, int Ns, int nvec, int vstride, int estride,
+                                              const cpx<T>* lo, const cpx<T>* hi, int tid, int nthr) {
+    const int total = (n / R) * nvec;
+    const int tstep = n / (Ns * R);
+    const float inv_nvec = 1.0f / (float)nvec, inv_ns = 1.0f / (float)Ns;
+    for (int idx = tid; idx < total; idx += nthr) {
+        int v, j;
+        const int bf = fast_div(idx, nvec, inv_nvec, v);
+        const int g = fast_div(bf, Ns, inv_ns, j);
+        const int base = (g * Ns * R + j) * estride + v * vstride;
+        cpx<T> a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = buf[base + r * Ns * estride];
+        if constexpr (R % 2 == 1 && R >= 11) {
+            auto sink = [&](int k, cpx<T> x) {
+                if (k && j) {
+                    const cpx<T> w = tw2(lo, hi, j * k * tstep);
+                    x = INV ? cmulc(x, w) : cmul(x, w);
+                }
+                buf[base + k * Ns * estride] = x;
+            };
+            Dft<T, R, INV>::run_sink(a, sink);
+        } else {
+            Dft<T, R, INV>::run(a);
+            if (j) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const cpx<T> w = tw2(lo, hi, j * r * tstep);
+                    a[r] = INV ? cmulc(a[r], w) : cmul(a[r], w);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[base + r * Ns * estride] = a[r];
+        }
+    }
+}
+
+template <typename T, bool INV, int MAXR = 71>
+__device__ __forceinline__ void fft_dit_lds_tw2v(cpx<T>* buf, const FftPlan& p, int nvec, int vstride, int estride,
+                                                 const cpx<T>* lo, const cpx<T>* hi, int tid, int nthr) {
+    int Ns = 1;
+    for (int s = 0; s < p.npass; ++s) {
+        const int R = p.radix[s];
+        switch (R) {
+#define S1G_CASE(r)                                                                                          \
+    case r:                                                                                                  \
+        if constexpr (r <= MAXR) dit_pass_tw2v<T, r, INV>(buf, p.n, Ns, nvec, vstride, estride, lo, hi, tid, nthr); \
+        break;
+            S1G_RADIX_LIST(S1G_CASE)
+#undef S1G_CASE
+            default:
+                break;
+        }
+        Ns *= R;
+        __syncthreads();
+    }
+}
+
+template <typename T, bool INV, int MAXR = 71>
+__device__ __forceinline__ void fft_dif_lds_tw2v(cpx<T>* buf, const FftPlan& p, int nvec, int vstride, int estride,
+                                                 const cpx<T>* lo, const cpx<T>* hi, int tid, int nthr) {
+    int Ns = p.n;
+    for (int s = p.npass - 1; s >= 0; --s) {
+        const int R = p.radix[s];
+        Ns /= R;
+        switch (R) {
+#define S1G_CASE(r)                                                                                          \
+    case r:                                                                                                  \
+        if constexpr (r <= MAXR) dif_pass_tw2v<T, r, INV>(buf, p.n, Ns, nvec, vstride, estride, lo, hi, tid, nthr); \
+        break;
+            S1G_RADIX_LIST(S1G_CASE)
+#undef S1G_CASE
+            default:
+                break;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace s1g
