@@ -231,9 +231,13 @@ def _rfi_lines(seed=21, L=24, N=3000):
     return x, det, 2000.0
 
 
+# N = 3000: the detection spectrum in one sub-spectrum; 21600 and 20400 (IW1
+# and IW3 lines): four (FP32) or eight (FP64) sub-spectra, 20400 with a
+# radix-17 pass.
+@pytest.mark.parametrize("N", [3000, 21600, 20400])
 @pytest.mark.parametrize("prec", [F64, F32])
-def test_eccm_vs_oracle(ctx, prec):
-    x, det, mit = _rfi_lines()
+def test_eccm_vs_oracle(ctx, prec, N):
+    x, det, mit = _rfi_lines(L=24 if N == 3000 else 12, N=N)
     ref, rflags = oracle.eccm(x, det, mit)
     dt = torch.complex128 if prec == F64 else torch.complex64
     d = torch.from_numpy(x).to(dt).cuda()
@@ -242,7 +246,7 @@ def test_eccm_vs_oracle(ctx, prec):
     ctx.eccm(d, desc, flags)
     torch.cuda.synchronize()
     assert np.array_equal(flags.cpu().numpy(), rflags)
-    assert rflags.sum() == 8
+    assert rflags.sum() == x.shape[0] // 3
     tol = 1e-12 if prec == F64 else 1e-5
     assert rel_l2(d.cpu().numpy(), ref) < tol
 
