@@ -132,8 +132,11 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
     const int w0 = max(chunk * wpc - ((fsz + stride - 1) / stride - 1), 0);
     const int w1 = min((chunk + 1) * wpc, nf);
     const int nw = w1 - w0;
-    cpx<T>* buf = (cpx<T>*)smem_raw;                       // nw * fsz
-    uint8_t* keep = (uint8_t*)(buf + (size_t)nw * fsz);    // nw * fsz
+    // window w at buf + w * vs: one pad element per window, so that lanes on
+    // neighbouring windows (the passes' fastest index) hit different banks
+    const int vs = fsz + 1;
+    cpx<T>* buf = (cpx<T>*)smem_raw;                       // nw * vs
+    uint8_t* keep = (uint8_t*)(buf + (size_t)nw * vs);     // nw * fsz
     // the window samples, loaded four per thread at a time before their LDS
     // stores (one memory wait per group)
     for (int i0 = 0; i0 < nw * fsz; i0 += 4 * NT) {
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
             v[u] = mk((T)0, (T)0);
             pos[u] = -1;
             if (idx < nw * fsz) {
-                pos[u] = w * fsz + perm[j];
+                pos[u] = w * vs + perm[j];
                 if (s + j < e) v[u] = row[s + j];
             }
         }
@@ -158,10 +161,10 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
             if (pos[u] >= 0) buf[pos[u]] = v[u];
     }
     __syncthreads();
-    fft_dit_lds<T, false, true>(buf, pw, nw, fsz, 1, tw, tid, NT);
+    fft_dit_lds<T, false, true>(buf, pw, nw, vs, 1, tw, tid, NT);
     for (int idx = tid; idx < nw * fsz; idx += NT) {
         const int w = idx / fsz, j = idx - w * fsz;
-        const cpx<T>* X = buf + w * fsz;
+        const cpx<T>* X = buf + w * vs;
         cpx<T> g;
         if (j != 0 && j != fsz - 1) {
             const cpx<T> d = X[j + 1] - X[j - 1];
@@ -175,15 +178,15 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
     }
     __syncthreads();
     for (int idx = tid; idx < nw * fsz; idx += NT)
-        if (!keep[idx]) buf[idx] = mk((T)0, (T)0);
+        if (!keep[idx]) buf[idx + idx / fsz] = mk((T)0, (T)0);  // window idx / fsz, bin idx mod fsz
     __syncthreads();
-    fft_dif_lds<T, true, true>(buf, pw, nw, fsz, 1, tw, tid, NT);
+    fft_dif_lds<T, true, true>(buf, pw, nw, vs, 1, tw, tid, NT);
     const T inv = (T)(1.0 / (double)fsz);
     for (int n = s0 + tid; n < s1; n += NT) {
         const int o = eccm_owner(n, N, fsz, stride, nf);
         if (o >= w0 && o < w1) {
             const int j = n - o * stride;
-            orow[n] = scl(buf[(o - w0) * fsz + perm[j]], inv);
+            orow[n] = scl(buf[(o - w0) * vs + perm[j]], inv);
         } else {
             orow[n] = row[n];
         }
@@ -214,7 +217,7 @@ static int run_eccm(hipStream_t s, const s1g_eccm_desc& d, const DevFft& fN, con
     const int wpc = 16;
     const int nchunk = (N + wpc * d.stride - 1) / (wpc * d.stride);
     const int nw_max = wpc + (d.fft_size + d.stride - 1) / d.stride - 1;
-    const size_t lds = (sizeof(cpx<T>) + 1) * (size_t)nw_max * d.fft_size;
+    const size_t lds = sizeof(cpx<T>) * (size_t)nw_max * (d.fft_size + 1) + (size_t)nw_max * d.fft_size;
     hipLaunchKernelGGL((eccm_notch_kernel<T, NT>), dim3((unsigned)(L * nchunk)), dim3(NT), lds, s,
                        (const cpx<T>*)in, ld_in, (cpx<T>*)out, ld_out, N, d.fft_size, d.stride, nf > 0 ? nf : 0, wpc,
                        nchunk, d.detection_threshold, d.mitigation_threshold, (const T*)peak, flags, fw.plan,
