@@ -75,6 +75,9 @@ struct s1g_ctx {
     std::vector<hipEvent_t> event_pool;
     // batches: the stream decodes run on, and ordering events
     hipStream_t side = nullptr;
+    // the long packets' decode class runs here beside the short ones
+    hipStream_t dec_side = nullptr;
+    hipEvent_t dec_fork = nullptr, dec_join = nullptr;
     std::vector<hipEvent_t> sync_events;
     struct Mark {
         int stage;
@@ -635,6 +638,9 @@ int s1g_destroy(s1g_ctx* c) {
     }
     for (auto e : c->event_pool) hipEventDestroy(e);
     if (c->side) hipStreamSynchronize(c->side);
+    if (c->dec_side) hipStreamSynchronize(c->dec_side);
+    if (c->dec_fork) hipEventDestroy(c->dec_fork);
+    if (c->dec_join) hipEventDestroy(c->dec_join);
     for (auto e : c->sync_events) hipEventDestroy(e);
     for (auto& kv : c->graphs) {
         hipGraphExecDestroy(kv.second.exec);
@@ -660,6 +666,7 @@ int s1g_destroy(s1g_ctx* c) {
     for (int k = 0; k < 16; ++k) hipFree(c->aux[k]);
     if (c->own) hipStreamDestroy(c->own);
     if (c->side) hipStreamDestroy(c->side);
+    if (c->dec_side) hipStreamDestroy(c->dec_side);
     delete c;
     return S1G_OK;
 }
@@ -918,8 +925,32 @@ int s1g_decode_fdbaq(s1g_ctx* c, const uint8_t* ud, const uint64_t* offs, const 
     if (!ud || !offs || !lens || !nq || !out || !status) return fail(c, S1G_EARG, "null pointer%s");
     if (ld_out < 2 * (int64_t)max_quads) return fail(c, S1G_EARG, "ld_out < 2 * max_num_quads%s");
     StageTimer tm(c, S1G_STAGE_DECODE);
-    int rc = launch_fdbaq_decode(c->stream, prec, ud, offs, lens, nq, n_packets, max_bytes, max_quads, out, ld_out,
+    int rc;
+    if (max_bytes <= kDecodeSplitBytes + 4096) {
+        rc = launch_fdbaq_decode(c->stream, prec, ud, offs, lens, nq, n_packets, max_bytes, max_quads, out, ld_out,
                                  status);
+    } else {
+        // A workgroup's LDS is sized for the launch's largest packet, which
+        // sets how many packets a CU holds.  With a few long packets (RFI
+        // lines: ~44 KB of mostly BRC-4 codes against ~19 KB) the packets up
+        // to kDecodeSplitBytes decode with LDS for that size, and the longer
+        // ones at once on a second stream (default priority), so both
+        // classes are resident together; the context stream waits for both.
+        if (!c->dec_side) HIPCHK(c, hipStreamCreateWithFlags(&c->dec_side, hipStreamNonBlocking));
+        if (!c->dec_fork) HIPCHK(c, hipEventCreateWithFlags(&c->dec_fork, hipEventDisableTiming));
+        if (!c->dec_join) HIPCHK(c, hipEventCreateWithFlags(&c->dec_join, hipEventDisableTiming));
+        HIPCHK(c, hipEventRecord(c->dec_fork, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->dec_side, c->dec_fork, 0));
+        rc = launch_fdbaq_decode(c->dec_side, prec, ud, offs, lens, nq, n_packets, max_bytes, max_quads, out, ld_out,
+                                 status, (uint32_t)kDecodeSplitBytes, 0xFFFFFFFFu);
+        const int rc2 = launch_fdbaq_decode(c->stream, prec, ud, offs, lens, nq, n_packets, kDecodeSplitBytes,
+                                            max_quads, out, ld_out, status, 0u, (uint32_t)kDecodeSplitBytes);
+        // join even after a failed launch, so the side stream never runs on
+        // unordered with later work of the context stream
+        HIPCHK(c, hipEventRecord(c->dec_join, c->dec_side));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->dec_join, 0));
+        if (!rc) rc = rc2;
+    }
     if (rc == -1) return fail(c, S1G_EUNSUPPORTED, "packet larger than the on-chip decode buffer%s");
     if (rc) return hip_fail(c, (hipError_t)rc, "fdbaq decode");
     return S1G_OK;

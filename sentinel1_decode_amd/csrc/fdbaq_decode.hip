@@ -609,7 +609,8 @@ __global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint
                                                            const int32_t* __restrict__ nqs, int n_packets,
                                                            int max_words, int lds_words, int max_blocks,
                                                            int max_quads, T* __restrict__ out, int64_t ld_out,
-                                                           int32_t* __restrict__ status) {
+                                                           int32_t* __restrict__ status, uint32_t lo_bytes,
+                                                           uint32_t hi_bytes) {
     constexpr int kThreads = kDecodeThreads;
     extern __shared__ __align__(16) uint32_t smem[];  // base 16-aligned after the static words
     __shared__ int s_err;          // first error status
@@ -622,6 +623,12 @@ __global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint
     // control flow rather than exec-masked
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (p >= n_packets) return;
+    // this launch's size class: lo_bytes < length <= hi_bytes (another
+    // launch of the burst takes the other packets)
+    {
+        const uint32_t len = lens[p];
+        if (len <= lo_bytes || len > hi_bytes) return;
+    }
 
     uint32_t* staged = smem;                                     // lds_words
     int32_t* bstart = (int32_t*)(smem + lds_words);              // 4 * max_blocks
@@ -889,10 +896,12 @@ __global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint
     if (tid == 0) status[p] = S1G_PKT_OK;
 }
 
-int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
-                        const int32_t* nqs, int n_packets, int max_bytes, int max_quads, void* out, int64_t ld_out,
-                        int32_t* status) {
-    if (n_packets <= 0) return 0;
+// One launch over the packets whose length l satisfies lo < l <= hi, its
+// LDS sized for max_bytes.
+static int launch_decode_class(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs,
+                               const uint32_t* lens, const int32_t* nqs, int n_packets, int max_bytes,
+                               int max_quads, void* out, int64_t ld_out, int32_t* status, uint32_t lo,
+                               uint32_t hi) {
     const int max_words = (max_bytes + 3) / 4;
     const int lds_words = std::min(max_words + kPadWords, kLdsWords);
     const int max_blocks = (max_quads + 127) / 128 + 1;
@@ -905,7 +914,7 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     const bool all = max_words + kPadWords <= kLdsWords;
 #define S1G_LAUNCH(T, A)                                                                                       \
     hipLaunchKernelGGL((fdbaq_decode_kernel<T, A>), grid, block, lds, stream, ud, offs, lens, nqs, n_packets, \
-                       max_words, lds_words, max_blocks, max_quads, (T*)out, ld_out, status)
+                       max_words, lds_words, max_blocks, max_quads, (T*)out, ld_out, status, lo, hi)
     if (prec == S1G_F32) {
         if (all) S1G_LAUNCH(float, true);
         else S1G_LAUNCH(float, false);
@@ -915,6 +924,14 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
     }
 #undef S1G_LAUNCH
     return (int)hipGetLastError();
+}
+
+int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
+                        const int32_t* nqs, int n_packets, int max_bytes, int max_quads, void* out, int64_t ld_out,
+                        int32_t* status, uint32_t lo_bytes, uint32_t hi_bytes) {
+    if (n_packets <= 0) return 0;
+    return launch_decode_class(stream, prec, ud, offs, lens, nqs, n_packets, max_bytes, max_quads, out, ld_out,
+                               status, lo_bytes, hi_bytes);
 }
 
 }  // namespace s1g

@@ -87,9 +87,14 @@ int ctx_dft_rows(s1g_ctx* c, int prec, int rows, int n, void* data, int64_t ld, 
 // last variant.
 int ctx_rcmc_variant(s1g_ctx* c, int prec, int n_lines, int n_samples, int apply_rcmc, int fallback);
 
+// Decode the packets whose length l satisfies lo_bytes < l <= hi_bytes (the
+// others are left to another launch), LDS sized for max_bytes.
 int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const uint64_t* offs, const uint32_t* lens,
                         const int32_t* nqs, int n_packets, int max_bytes, int max_quads, void* out, int64_t ld_out,
-                        int32_t* status);
+                        int32_t* status, uint32_t lo_bytes = 0u, uint32_t hi_bytes = 0xFFFFFFFFu);
+// Packets up to this length decode with LDS sized for it when a burst also
+// holds longer ones (launch_fdbaq_decode per size class).
+constexpr int kDecodeSplitBytes = 24 * 1024;
 
 // Out-of-place Stockham FFT of `batch` vectors of length n (vector b at
 // in + b*dist) in global memory, one launch per pass; the result ends in

@@ -138,3 +138,50 @@ def test_rcmc_fp32_offset_rounding_to_one(ctx, k, pixels):
             assert abs(complex(y[i, j]) - complex(ref[i, j])) < 1e-4 * abs(complex(ref[i, j])) + 1e-3, (v, i, j)
         err = float((y.to(torch.complex128) - ref).norm() / ref.norm())
         assert err < 2e-6, (v, err)
+
+
+@pytest.fixture(scope="module")
+def rfi_burst():
+    """64 lines of 24,000 samples with strong narrowband RFI (30-40 dB INR)
+    on about a fifth of them: the RFI lines' packets (36-48 KB) are longer
+    than the decode's short-packet class (24 KB), the others (~19 KB)
+    shorter."""
+    spec = synth.config1(n_lines=64, n_targets=20, seed=2011)
+    raw = spec.raw()
+    synth.add_rfi(raw, spec, seed=2011, frac=0.2, run=(4, 8), inr_db=(30.0, 40.0))
+    return synth.encode_burst(spec, raw)
+
+
+def test_decode_size_classes_bit_exact(ctx, rfi_burst):
+    """Packets of both length classes (decoded by two launches on two
+    streams) decode bit-exactly, FP64 and FP32."""
+    lens = np.asarray(rfi_burst.ud_lengths)
+    assert lens.max() > 28 * 1024 and lens.min() <= 24 * 1024, (lens.min(), lens.max())
+    ref, rst = oracle.decode_packets(rfi_burst.stream, rfi_burst.ud_offsets, rfi_burst.ud_lengths,
+                                     rfi_burst.num_quads)
+    assert (np.asarray(rst) == 0).all()
+    db = pipeline.from_synth(rfi_burst)
+    for prec in (capi.S1G_F64, capi.S1G_F32):
+        dec, st = db.decode(ctx, prec)
+        torch.cuda.synchronize()
+        assert (st.cpu().numpy() == 0).all(), prec
+        got = dec.cpu().numpy()
+        want = ref if prec == capi.S1G_F64 else ref.astype(np.complex64)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), prec
+
+
+def test_decode_size_classes_in_batch_graph(ctx, rfi_burst):
+    """The two-stream decode inside a captured batch: two copies of the burst
+    in one graph, replayed twice, equal the single eager focus."""
+    db = pipeline.from_synth(rfi_burst)
+    eager = db.focus(ctx, capi.S1G_F32)
+    torch.cuda.synchronize()
+    outs = [torch.zeros_like(eager), torch.zeros_like(eager)]
+    descs = [db.burst_desc(o) for o in outs]
+    for rep in range(2):
+        for o in outs:
+            o.zero_()
+        ctx.focus_batch(descs, capi.S1G_F32, graph=True)
+        torch.cuda.synchronize()
+        for o in outs:
+            assert torch.equal(o, eager), rep
