@@ -94,7 +94,7 @@ int launch_fdbaq_decode(hipStream_t stream, int prec, const uint8_t* ud, const u
                         int32_t* status, uint32_t lo_bytes = 0u, uint32_t hi_bytes = 0xFFFFFFFFu);
 // Packets up to this length decode with LDS sized for it when a burst also
 // holds longer ones (launch_fdbaq_decode per size class).
-constexpr int kDecodeSplitBytes = 24 * 1024;
+constexpr int kDecodeSplitBytes = 20 * 1024;
 
 // Out-of-place Stockham FFT of `batch` vectors of length n (vector b at
 // in + b*dist) in global memory, one launch per pass; the result ends in

@@ -144,7 +144,7 @@ def test_rcmc_fp32_offset_rounding_to_one(ctx, k, pixels):
 def rfi_burst():
     """64 lines of 24,000 samples with strong narrowband RFI (30-40 dB INR)
     on about a fifth of them: the RFI lines' packets (36-48 KB) are longer
-    than the decode's short-packet class (24 KB), the others (~19 KB)
+    than the decode's short-packet class (20 KB), the others (~19 KB)
     shorter."""
     spec = synth.config1(n_lines=64, n_targets=20, seed=2011)
     raw = spec.raw()
@@ -156,7 +156,7 @@ def test_decode_size_classes_bit_exact(ctx, rfi_burst):
     """Packets of both length classes (decoded by two launches on two
     streams) decode bit-exactly, FP64 and FP32."""
     lens = np.asarray(rfi_burst.ud_lengths)
-    assert lens.max() > 28 * 1024 and lens.min() <= 24 * 1024, (lens.min(), lens.max())
+    assert lens.max() > 24 * 1024 and lens.min() <= 20 * 1024, (lens.min(), lens.max())
     ref, rst = oracle.decode_packets(rfi_burst.stream, rfi_burst.ud_offsets, rfi_burst.ud_lengths,
                                      rfi_burst.num_quads)
     assert (np.asarray(rst) == 0).all()
