@@ -69,10 +69,11 @@ def test_batch_shared_output_runs_in_order(ctx, uneven_bursts, graph):
 
 
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_with_eccm_matches_single_bursts(ctx, uneven_bursts, graph):
+@pytest.mark.parametrize("eccm_first", [True, False])
+def test_batch_with_eccm_matches_single_bursts(ctx, uneven_bursts, graph, eccm_first):
     """A batch in which one burst runs ECCM (its decode goes through the
     shared workspace, so the chains run in order) gives each burst the image
-    and flags of its own single call."""
+    and flags of its own single call, with the ECCM burst first or second."""
     a, b = uneven_bursts[1], uneven_bursts[0]
     ed = capi.EccmDesc(a.n_lines, a.n_samples, 128, 64, 1.0e3, 4000.0)
     fa = torch.zeros(a.n_lines, dtype=torch.uint8, device="cuda")
@@ -82,6 +83,8 @@ def test_batch_with_eccm_matches_single_bursts(ctx, uneven_bursts, graph):
     outs = [torch.zeros_like(ea), torch.zeros_like(eb)]
     flags = torch.full_like(fa, 7)
     descs = [a.burst_desc(outs[0], eccm=ed, eccm_flags=flags), b.burst_desc(outs[1])]
+    if not eccm_first:
+        descs = descs[::-1]
     for rep in range(2):
         for o in outs:
             o.zero_()

@@ -161,27 +161,41 @@ def test_decode_size_classes_bit_exact(ctx, rfi_burst):
                                      rfi_burst.num_quads)
     assert (np.asarray(rst) == 0).all()
     db = pipeline.from_synth(rfi_burst)
-    for prec in (capi.S1G_F64, capi.S1G_F32):
+    for prec in (F64, F32):
         dec, st = db.decode(ctx, prec)
         torch.cuda.synchronize()
         assert (st.cpu().numpy() == 0).all(), prec
         got = dec.cpu().numpy()
-        want = ref if prec == capi.S1G_F64 else ref.astype(np.complex64)
+        want = ref if prec == F64 else ref.astype(np.complex64)
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), prec
 
 
 def test_decode_size_classes_in_batch_graph(ctx, rfi_burst):
-    """The two-stream decode inside a captured batch: two copies of the burst
-    in one graph, replayed twice, equal the single eager focus."""
+    """The two-stream decode inside a captured batch of three copies of the
+    burst, the first and last with ECCM (so the chains run in order, each
+    decode forking its long packets onto the decode stream and joining
+    back).  Replayed twice, every image and flag vector equals the burst's
+    single eager call."""
     db = pipeline.from_synth(rfi_burst)
-    eager = db.focus(ctx, capi.S1G_F32)
+    L, N = rfi_burst.spec.n_lines, rfi_burst.spec.n_samples
+    ed = capi.EccmDesc(L, N, 128, 64, 1.0e5, 4000.0)
+    f_ref = torch.zeros(L, dtype=torch.uint8, device="cuda")
+    e_eccm = db.focus(ctx, F32, eccm=ed, eccm_flags=f_ref)
+    e_plain = db.focus(ctx, F32)
     torch.cuda.synchronize()
-    outs = [torch.zeros_like(eager), torch.zeros_like(eager)]
-    descs = [db.burst_desc(o) for o in outs]
+    assert 0 < int(f_ref.sum()) < L
+    assert not torch.equal(e_eccm, e_plain)
+    outs = [torch.zeros_like(e_plain) for _ in range(3)]
+    flags = [torch.full_like(f_ref, 7), torch.full_like(f_ref, 7)]
+    descs = [db.burst_desc(outs[0], eccm=ed, eccm_flags=flags[0]), db.burst_desc(outs[1]),
+             db.burst_desc(outs[2], eccm=ed, eccm_flags=flags[1])]
     for rep in range(2):
         for o in outs:
             o.zero_()
-        ctx.focus_batch(descs, capi.S1G_F32, graph=True)
+        ctx.focus_batch(descs, F32, graph=True)
         torch.cuda.synchronize()
-        for o in outs:
-            assert torch.equal(o, eager), rep
+        assert torch.equal(outs[0], e_eccm), rep
+        assert torch.equal(outs[1], e_plain), rep
+        assert torch.equal(outs[2], e_eccm), rep
+        for f in flags:
+            assert torch.equal(f, f_ref), rep
