@@ -531,10 +531,17 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
 // bits per block; the margin grows with the number of blocks predicted over.
 constexpr int kSpecMargin = 40;
 constexpr int kSpecMarginStep = 24;
-// Waves that walk the IO, QE and QO blocks, block b on wave b mod kWalkers;
-// one more wave decodes values.
+// Waves that walk the IO, QE and QO blocks, block b on wave b mod W; one
+// more wave decodes values.  kWalkers for a launch over all of a burst's
+// packets (or its short ones): four waves per packet keep 7 packets per CU.
+// A launch over only the long packets of a burst (a few hundred RFI lines,
+// about one packet per CU) is bound by each packet's walk latency, and more
+// walking waves shorten the pipelined channels' walks: on a 9-burst IW2
+// swath with ~300 RFI lines per burst, 3, 4, 5, 7, 9, 11 and 15 walkers gave
+// 22.7, 21.9, 21.8, 21.6, 21.5, 21.6 and 25.5 ms.  With nearly every packet
+// long (LDS-bound at 3 packets per CU) 7 walkers cost 3.5% against 3.
 constexpr int kWalkers = 3;
-constexpr int kDecodeThreads = 64 * (kWalkers + 1);
+constexpr int kWalkersLong = 7;
 // Progress value of a finished (or abandoned) channel.
 constexpr int kDone = 1 << 20;
 
@@ -582,10 +589,10 @@ __device__ __forceinline__ bool wait_start(int* flag, int need, int* err, int la
     return false;
 }
 
-// kWalkers + 1 wavefronts per packet.
+// W + 1 wavefronts per packet.
 //   wave 0 walks IE block by block (each block's BRC is in the stream);
-//   waves 0 .. kWalkers - 1 then walk IO, QE and QO as a pipeline: block b
-//     is walked by wave b mod kWalkers.  Its BRC is known from IE, so before
+//   waves 0 .. W - 1 then walk IO, QE and QO as a pipeline: block b
+//     is walked by wave b mod W.  Its BRC is known from IE, so before
 //     its start is known the wave speculates: it walks and settles a region
 //     from the latest block start it knows (the block after the one it last
 //     resolved) plus IE's lengths of the blocks in between, less a margin
@@ -593,8 +600,8 @@ __device__ __forceinline__ bool wait_start(int* flag, int need, int* err, int la
 //     start (an LDS word with release / acquire at workgroup scope), the
 //     chain from there is the settled chain whenever the start is one of its
 //     recorded symbol starts -- one ballot -- so each block costs about
-//     1/kWalkers of a walk on the critical path;
-//   wave kWalkers decodes values, in groups of 16 blocks (lane 4j + ch takes
+//     1/W of a walk on the critical path;
+//   wave W decodes values, in groups of 16 blocks (lane 4j + ch takes
 //     channel ch of the group's block j, so each store instruction writes
 //     whole samples), as soon as the group's QO starts are published; the
 //     walking waves join it when the walks end.
@@ -602,8 +609,8 @@ __device__ __forceinline__ bool wait_start(int* flag, int need, int* err, int la
 // before it), posted once, and end every wait.  The walks are the packet's
 // critical path, so the walking waves raise their issue priority
 // (s_setprio) over the value decodes sharing their SIMDs.
-template <typename T, bool ALL>
-__global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
+template <typename T, bool ALL, int W>
+__global__ __launch_bounds__(64 * (W + 1)) void fdbaq_decode_kernel(const uint8_t* __restrict__ ud,
                                                            const uint64_t* __restrict__ offs,
                                                            const uint32_t* __restrict__ lens,
                                                            const int32_t* __restrict__ nqs, int n_packets,
@@ -611,7 +618,7 @@ __global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint
                                                            int max_quads, T* __restrict__ out, int64_t ld_out,
                                                            int32_t* __restrict__ status, uint32_t lo_bytes,
                                                            uint32_t hi_bytes) {
-    constexpr int kThreads = kDecodeThreads;
+    constexpr int kThreads = 64 * (W + 1);
     extern __shared__ __align__(16) uint32_t smem[];  // base 16-aligned after the static words
     __shared__ int s_err;          // first error status
     __shared__ int s_pub[4];       // blocks of each channel whose start is published; kDone when finished
@@ -763,8 +770,8 @@ __global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint
         __builtin_amdgcn_s_setprio(0);
     }
 
-    if (wave < kWalkers) {
-        // ---- IO, QE, QO: block b on wave b mod kWalkers
+    if (wave < W) {
+        // ---- IO, QE, QO: block b on wave b mod W
         __builtin_amdgcn_s_setprio(2);
         for (int ch = 1; ch < 4; ++ch) {
             if (!wait_for<4>(&s_pub[ch - 1], kDone, &s_err, lane)) break;
@@ -798,7 +805,7 @@ __global__ __launch_bounds__(kDecodeThreads) void fdbaq_decode_kernel(const uint
             // the latest block start this wave knows: block kb starts at base
             int base = s0, kb = 0;
             bool stop = false;
-            for (int b = wave; b < nb && !stop; b += kWalkers) {
+            for (int b = wave; b < nb && !stop; b += W) {
                 const int brc = __builtin_amdgcn_readfirstlane((int)bbrc[b]);
                 const int nsym = (b == nb - 1) ? nq - 128 * (nb - 1) : 128;
                 uint32_t hdr = 0;
@@ -910,18 +917,28 @@ static int launch_decode_class(hipStream_t stream, int prec, const uint8_t* ud, 
     const int nrl_words = (lds_words + 4 * max_blocks + (2 * max_blocks + 3) / 4 + 1) & ~1;
     const size_t lds = sizeof(uint32_t) * (size_t)nrl_words + sizeof(double) * 80;
     if (lds > 160 * 1024) return -1;
-    dim3 grid(n_packets), block(kDecodeThreads);
     const bool all = max_words + kPadWords <= kLdsWords;
-#define S1G_LAUNCH(T, A)                                                                                       \
-    hipLaunchKernelGGL((fdbaq_decode_kernel<T, A>), grid, block, lds, stream, ud, offs, lens, nqs, n_packets, \
-                       max_words, lds_words, max_blocks, max_quads, (T*)out, ld_out, status, lo, hi)
+    // the long class of a split burst (lo > 0) takes more walking waves
+    const int walkers = lo > 0 ? kWalkersLong : kWalkers;
+#define S1G_LAUNCH(T, A, W)                                                                                    \
+    hipLaunchKernelGGL((fdbaq_decode_kernel<T, A, W>), dim3(n_packets), dim3(64 * (W + 1)), lds, stream, ud, offs, \
+                       lens, nqs, n_packets, max_words, lds_words, max_blocks, max_quads, (T*)out, ld_out, status,   \
+                       lo, hi)
+#define S1G_LAUNCH_W(T, A)                      \
+    do {                                        \
+        if (walkers == kWalkersLong)            \
+            S1G_LAUNCH(T, A, kWalkersLong);     \
+        else                                    \
+            S1G_LAUNCH(T, A, kWalkers);         \
+    } while (0)
     if (prec == S1G_F32) {
-        if (all) S1G_LAUNCH(float, true);
-        else S1G_LAUNCH(float, false);
+        if (all) S1G_LAUNCH_W(float, true);
+        else S1G_LAUNCH_W(float, false);
     } else {
-        if (all) S1G_LAUNCH(double, true);
-        else S1G_LAUNCH(double, false);
+        if (all) S1G_LAUNCH_W(double, true);
+        else S1G_LAUNCH_W(double, false);
     }
+#undef S1G_LAUNCH_W
 #undef S1G_LAUNCH
     return (int)hipGetLastError();
 }
