@@ -539,7 +539,7 @@ int do_rcmc(s1g_ctx* c, const s1g_azimuth_desc& d, int prec, const void* in, int
 }
 
 int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_t ld_in, void* out, int64_t ld_out,
-            uint8_t* flags) {
+            uint8_t* flags, void* stage = nullptr, int64_t ld_stage = 0) {
     const int L = d.n_lines, N = d.n_samples;
     if (L <= 0 || N <= 0) return L < 0 || N < 0 ? fail(c, S1G_EARG, "bad ECCM shape%s") : 0;
     const int fsz = d.fft_size;
@@ -559,7 +559,7 @@ int do_eccm(s1g_ctx* c, const s1g_eccm_desc& d, int prec, const void* in, int64_
     if (rc) return rc;
     StageTimer tm(c, S1G_STAGE_ECCM);
     return check_launch(c, launch_eccm(c->stream, prec, d, *fN, *fsub, r, *fw, in, ld_in, out, ld_out, flags,
-                                       c->eccm_scratch),
+                                       c->eccm_scratch, stage, ld_stage),
                         "eccm");
 }
 
@@ -1012,11 +1012,9 @@ int s1g_eccm(s1g_ctx* c, const s1g_eccm_desc* d, int prec, void* data, int64_t l
     const size_t row = esize(prec) * (size_t)d->n_samples;
     int rc = ensure(c, &c->ws, &c->ws_bytes, row * d->n_lines);
     if (rc) return rc;
-    rc = do_eccm(c, *d, prec, data, ld, c->ws, d->n_samples, flags);
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpy2DAsync(data, esize(prec) * ld, c->ws, row, row, d->n_lines, hipMemcpyDeviceToDevice,
-                               c->stream));
-    return S1G_OK;
+    // in place: flagged lines are staged in the workspace and notched back
+    // into data; unflagged lines are left as they are
+    return do_eccm(c, *d, prec, data, ld, data, ld, flags, c->ws, d->n_samples);
 }
 
 // ------------------------------------------------------------ general DSP
@@ -1237,12 +1235,13 @@ static int burst_decode(s1g_ctx* c, const s1g_burst_desc& b, int prec) {
     int rc = ensure(c, &c->ws, &c->ws_bytes, esize(prec) * (size_t)L * N);
     if (rc) return rc;
     if (b.use_eccm) {
-        // decode -> workspace -> ECCM -> out (ECCM is out of place)
+        // decode -> out; ECCM on out's rows, its flagged lines staged in the
+        // workspace (unflagged lines stay as decoded)
         if (b.eccm.n_lines != L || b.eccm.n_samples != N) return fail(c, S1G_EARG, "ECCM shape disagrees%s");
         rc = s1g_decode_fdbaq(c, b.user_data, b.ud_offsets, b.ud_lengths, b.num_quads, L, b.max_user_data_bytes,
-                              b.max_num_quads, prec, c->ws, N, b.status);
+                              b.max_num_quads, prec, b.out, b.ld_out, b.status);
         if (rc) return rc;
-        rc = do_eccm(c, b.eccm, prec, c->ws, N, b.out, b.ld_out, b.eccm_flags);
+        rc = do_eccm(c, b.eccm, prec, b.out, b.ld_out, b.out, b.ld_out, b.eccm_flags, c->ws, N);
     } else {
         rc = s1g_decode_fdbaq(c, b.user_data, b.ud_offsets, b.ud_lengths, b.num_quads, L, b.max_user_data_bytes,
                               b.max_num_quads, prec, b.out, b.ld_out, b.status);

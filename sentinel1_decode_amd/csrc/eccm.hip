@@ -21,6 +21,10 @@
 // order of the bins, so natural-order input and DIF passes need no
 // digit-reversal.
 // Notch: one workgroup per (line, run of windows); unflagged lines are copied.
+// In a burst chain (in and out the same rows) the detection kernel also
+// copies each flagged line to a staging buffer, and the notch reads the
+// flagged lines from there and leaves the unflagged ones alone: the 80-90%
+// of lines without RFI are then neither read nor written again.
 #include "s1g_internal.h"
 #include "s1g_lds_fft.h"
 
@@ -29,7 +33,8 @@ namespace s1g {
 template <typename T, int NT, int MAXR>
 __global__ __launch_bounds__(NT) void eccm_detect_kernel(const cpx<T>* __restrict__ in, int64_t ld, int N, int r,
                                                          FftPlan sub, const cpx<T>* __restrict__ twN,
-                                                         const cpx<T>* __restrict__ twSub, T* __restrict__ peak) {
+                                                         const cpx<T>* __restrict__ twSub, T* __restrict__ peak,
+                                                         double det, cpx<T>* __restrict__ stage, int64_t ld_stage) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n1 = N / r;
     cpx<T>* buf = (cpx<T>*)smem_raw;   // n1
@@ -91,6 +96,28 @@ __global__ __launch_bounds__(NT) void eccm_detect_kernel(const cpx<T>* __restric
     if (tid == 0) {
         for (int w = 1; w < NT / 64; ++w) mx = fmax(mx, red[w]);
         peak[line] = sqrt(mx);
+        red[0] = mx;
+    }
+    if (stage) {
+        // a flagged line (the notch kernel's test on the same peak value)
+        // to the staging buffer, four loads per thread issued together
+        __syncthreads();
+        if (!((double)sqrt(red[0]) < det)) {
+            cpx<T>* srow = stage + (int64_t)line * ld_stage;
+            for (int n0 = 0; n0 < N; n0 += 4 * NT) {
+                cpx<T> v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int n = n0 + tid + NT * u;
+                    if (n < N) v[u] = row[n];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int n = n0 + tid + NT * u;
+                    if (n < N) srow[n] = v[u];
+                }
+            }
+        }
     }
 }
 
@@ -114,7 +141,7 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
                                                         double mit, const T* __restrict__ peak,
                                                         uint8_t* __restrict__ flags, FftPlan pw,
                                                         const cpx<T>* __restrict__ tw,
-                                                        const int32_t* __restrict__ perm) {
+                                                        const int32_t* __restrict__ perm, int copy_unflagged) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int line = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
     const int tid = threadIdx.x;
@@ -125,7 +152,8 @@ __global__ __launch_bounds__(NT) void eccm_notch_kernel(const cpx<T>* __restrict
     const int s0 = chunk * wpc * stride;
     const int s1 = min(s0 + wpc * stride, N);
     if (!flagged) {
-        for (int n = s0 + tid; n < s1; n += NT) orow[n] = row[n];
+        if (copy_unflagged)
+            for (int n = s0 + tid; n < s1; n += NT) orow[n] = row[n];
         return;
     }
     // windows whose samples can own [s0, s1): the run [w0, w1)
@@ -199,13 +227,14 @@ static size_t detect_lds_bytes(size_t es, int n1, int r) { return es * (size_t)(
 template <typename T>
 static int run_eccm(hipStream_t s, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,
                     const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
-                    void* peak) {
+                    void* peak, void* stage, int64_t ld_stage) {
     constexpr int NT = 512;
     const int L = d.n_lines, N = d.n_samples;
     const size_t lds_det = detect_lds_bytes(sizeof(cpx<T>), N / r, r);
 #define S1G_DETECT(CLS)                                                                                        \
     hipLaunchKernelGGL((eccm_detect_kernel<T, NT, kRadixClassMax[CLS]>), dim3((unsigned)L), dim3(NT), lds_det, s, \
-                       (const cpx<T>*)in, ld_in, N, r, fsub.plan, tw_of<T>(fN), tw_of<T>(fsub), (T*)peak)
+                       (const cpx<T>*)in, ld_in, N, r, fsub.plan, tw_of<T>(fN), tw_of<T>(fsub), (T*)peak,      \
+                       d.detection_threshold, (cpx<T>*)stage, ld_stage)
     switch (radix_class(fsub.plan)) {
         case 0: S1G_DETECT(0); break;
         case 1: S1G_DETECT(1); break;
@@ -218,10 +247,13 @@ static int run_eccm(hipStream_t s, const s1g_eccm_desc& d, const DevFft& fN, con
     const int nchunk = (N + wpc * d.stride - 1) / (wpc * d.stride);
     const int nw_max = wpc + (d.fft_size + d.stride - 1) / d.stride - 1;
     const size_t lds = sizeof(cpx<T>) * (size_t)nw_max * (d.fft_size + 1) + (size_t)nw_max * d.fft_size;
+    // staged: the notch reads the flagged lines from the staging buffer
+    const void* nin = stage ? stage : in;
+    const int64_t nld = stage ? ld_stage : ld_in;
     hipLaunchKernelGGL((eccm_notch_kernel<T, NT>), dim3((unsigned)(L * nchunk)), dim3(NT), lds, s,
-                       (const cpx<T>*)in, ld_in, (cpx<T>*)out, ld_out, N, d.fft_size, d.stride, nf > 0 ? nf : 0, wpc,
+                       (const cpx<T>*)nin, nld, (cpx<T>*)out, ld_out, N, d.fft_size, d.stride, nf > 0 ? nf : 0, wpc,
                        nchunk, d.detection_threshold, d.mitigation_threshold, (const T*)peak, flags, fw.plan,
-                       tw_of<T>(fw), fw.perm);
+                       tw_of<T>(fw), fw.perm, stage ? 0 : 1);
     return (int)hipGetLastError();
 }
 
@@ -240,9 +272,10 @@ int eccm_detect_split(int prec, int N, bool (*factor)(int, FftPlan&)) {
 
 int launch_eccm(hipStream_t s, int prec, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,
                  const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
-                 void* partial) {
-    if (prec == S1G_F32) return run_eccm<float>(s, d, fN, fsub, r, fw, in, ld_in, out, ld_out, flags, partial);
-    return run_eccm<double>(s, d, fN, fsub, r, fw, in, ld_in, out, ld_out, flags, partial);
+                 void* partial, void* stage, int64_t ld_stage) {
+    if (prec == S1G_F32)
+        return run_eccm<float>(s, d, fN, fsub, r, fw, in, ld_in, out, ld_out, flags, partial, stage, ld_stage);
+    return run_eccm<double>(s, d, fN, fsub, r, fw, in, ld_in, out, ld_out, flags, partial, stage, ld_stage);
 }
 
 }  // namespace s1g

@@ -186,9 +186,12 @@ int launch_rcmc_mf(hipStream_t stream, int prec, const s1g_azimuth_desc& d, cons
 // Out of place (windows overlap, so workgroups would race in place).
 // fN: plan of length N (twiddles only), fsub: plan of N / r, fw: window plan;
 // partial: n_lines values of the detection maxima.
+// With `stage` (rows of ld_stage elements, L of them): in and out may be the
+// same rows; flagged lines are copied to stage by the detection kernel and
+// notched from there into out, unflagged lines are not touched.
 int launch_eccm(hipStream_t stream, int prec, const s1g_eccm_desc& d, const DevFft& fN, const DevFft& fsub, int r,
                 const DevFft& fw, const void* in, int64_t ld_in, void* out, int64_t ld_out, uint8_t* flags,
-                void* partial);
+                void* partial, void* stage = nullptr, int64_t ld_stage = 0);
 // Detection split r for lines of N samples (0: none), given the planner.
 int eccm_detect_split(int prec, int N, bool (*factor)(int, FftPlan&));
 
