@@ -531,6 +531,11 @@ __device__ __forceinline__ void decode_item(const PacketWords<ALL>& words, const
 // bits per block; the margin grows with the number of blocks predicted over.
 constexpr int kSpecMargin = 40;
 constexpr int kSpecMarginStep = 24;
+// The long class (kWalkersLong walkers) predicts six blocks ahead of a block
+// it knows, over BRC-4 blocks of ~700 bits: on the 9-burst RFI swath margin
+// steps of 12, 24, 40, 56, 72 and 96 bits gave 21.6, 21.3, 20.9, 20.9, 20.9
+// and 20.8 ms.
+constexpr int kSpecMarginStepLong = 56;
 // Waves that walk the IO, QE and QO blocks, block b on wave b mod W; one
 // more wave decodes values.  kWalkers for a launch over all of a burst's
 // packets (or its short ones): four waves per packet keep 7 packets per CU.
@@ -820,7 +825,8 @@ __global__ __launch_bounds__(64 * (W + 1)) void fdbaq_decode_kernel(const uint8_
                     // words reach
                     int est = base;
                     for (int q = kb; q < b; ++q) est += ie_len(q) + hbits;
-                    const int margin = kSpecMargin + kSpecMarginStep * (b - kb - 1);
+                    const int margin =
+                        kSpecMargin + (W == kWalkersLong ? kSpecMarginStepLong : kSpecMarginStep) * (b - kb - 1);
                     const int pred = min(max(est - margin, base), nbits);
                     const int delta = min(max((ie_len(b) + 2 * margin + 16 + 63) / 64 + 1, 2), 32);
                     const uint32_t dmagic = 0xFFFFFFFFu / (uint32_t)delta + 1u;
